@@ -1,0 +1,533 @@
+// C ABI of libcurafft.so: the reference's four entry points over host buffers,
+// its C++ plan API, and the additive device-resident / FP32 / multi-channel exports.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/cugridder.h"
+#include "../../include/curig_ext.h"
+#include "curig_internal.h"
+#include "gridder.h"
+
+using namespace curig;
+
+namespace {
+
+std::mutex g_host_mu;
+Tuning g_tuning;
+bool g_stats_on = false;
+Stats g_last_stats;
+
+struct HostPath {
+  hipStream_t stream = nullptr;
+  void *buf[4] = {};
+  size_t cap[4] = {};
+};
+std::vector<HostPath> g_host;
+
+int host_buf(HostPath &h, int i, size_t bytes, void **out) {
+  if (bytes == 0) bytes = 16;
+  if (h.cap[i] < bytes) {
+    if (h.buf[i]) CURIG_HIP(hipFree(h.buf[i]));
+    h.buf[i] = nullptr;
+    h.cap[i] = 0;
+    hipError_t e = hipMalloc(&h.buf[i], bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("device allocation of %zu bytes failed", bytes);
+      return kErrAlloc;
+    }
+    h.cap[i] = bytes;
+  }
+  *out = h.buf[i];
+  return kOk;
+}
+
+int host_path(HostPath **out) {
+  int dev = 0;
+  CURIG_HIP(hipGetDevice(&dev));
+  if ((int)g_host.size() <= dev) g_host.resize(dev + 1);
+  HostPath &h = g_host[dev];
+  if (!h.stream) CURIG_HIP(hipStreamCreate(&h.stream));
+  *out = &h;
+  return kOk;
+}
+
+// Argument checks that need no device (same order of precedence as the reference:
+// an epsilon below 1.1e-16 returns 1 before anything is computed).
+int precheck(int nrow, int nx, int ny, double eps, double sigma) {
+  KernelOpts ko;
+  int ier = setup_conv_opts(ko, eps, sigma);
+  if (ier != kOk) return ier;
+  if (nrow < 0) {
+    set_error("nrow %d < 0", nrow);
+    return kErrNrow;
+  }
+  if (nx <= 0 || ny <= 0) {
+    set_error("image size %d x %d invalid", nx, ny);
+    return kErrSize;
+  }
+  return kOk;
+}
+
+Stats *stats_ptr() { return g_stats_on ? &g_last_stats : nullptr; }
+
+// Host buffers in -> device -> gridder -> host buffers out.
+int host_call(bool to_image, int nrow, int nx, int ny, double fov, double freq, const double *uvw, void *vis,
+              const void *wgt, void *dirty, double eps, double sigma, int sign, bool dbl) {
+  CURIG_TRY(precheck(nrow, nx, ny, eps, sigma));
+  if (!dirty || (nrow > 0 && (!uvw || !vis))) {
+    set_error("NULL data pointer");
+    return kErrArg;
+  }
+  std::lock_guard<std::mutex> lk(g_host_mu);
+  HostPath *h;
+  CURIG_TRY(host_path(&h));
+  const size_t cs = dbl ? 16 : 8, rs = dbl ? 8 : 4;
+  const size_t nr = (size_t)nrow, npix = (size_t)nx * ny;
+  void *d_uvw, *d_vis, *d_wgt = nullptr, *d_dirty;
+  CURIG_TRY(host_buf(*h, 0, 24 * nr, &d_uvw));
+  CURIG_TRY(host_buf(*h, 1, cs * nr, &d_vis));
+  CURIG_TRY(host_buf(*h, 3, cs * npix, &d_dirty));
+  hipStream_t st = h->stream;
+  if (nrow > 0) CURIG_HIP(hipMemcpyAsync(d_uvw, uvw, 24 * nr, hipMemcpyHostToDevice, st));
+  if (wgt && nrow > 0) {
+    CURIG_TRY(host_buf(*h, 2, rs * nr, &d_wgt));
+    CURIG_HIP(hipMemcpyAsync(d_wgt, wgt, rs * nr, hipMemcpyHostToDevice, st));
+  }
+  Problem pb;
+  pb.nrow = nrow;
+  pb.nx = nx;
+  pb.ny = ny;
+  pb.fov = fov;
+  pb.freq = freq;
+  pb.eps = eps;
+  pb.sigma = sigma;
+  pb.sign = sign;
+  pb.dbl = dbl;
+  pb.uvw = (const double *)d_uvw;
+  pb.vis = d_vis;
+  pb.wgt = d_wgt;
+  pb.dirty = d_dirty;
+  int ier;
+  if (to_image) {
+    if (nrow > 0) CURIG_HIP(hipMemcpyAsync(d_vis, vis, cs * nr, hipMemcpyHostToDevice, st));
+    ier = ms2dirty_device(pb, st, &g_tuning, stats_ptr());
+    if (ier != kOk) return ier;
+    CURIG_HIP(hipMemcpyAsync(dirty, d_dirty, cs * npix, hipMemcpyDeviceToHost, st));
+  } else {
+    CURIG_HIP(hipMemcpyAsync(d_dirty, dirty, cs * npix, hipMemcpyHostToDevice, st));
+    ier = dirty2ms_device(pb, st, &g_tuning, stats_ptr());
+    if (ier != kOk) return ier;
+    if (nrow > 0) CURIG_HIP(hipMemcpyAsync(vis, d_vis, cs * nr, hipMemcpyDeviceToHost, st));
+  }
+  CURIG_HIP(hipStreamSynchronize(st));
+  return kOk;
+}
+
+Problem dev_problem(int nrow, int nx, int ny, double fov, double freq, const double *uvw, const void *vis,
+                    const void *wgt, const void *dirty, double eps, double sigma, int sign, int precision) {
+  Problem pb;
+  pb.nrow = nrow;
+  pb.nx = nx;
+  pb.ny = ny;
+  pb.fov = fov;
+  pb.freq = freq;
+  pb.eps = eps;
+  pb.sigma = sigma;
+  pb.sign = sign;
+  pb.dbl = precision == 0;
+  pb.uvw = uvw;
+  pb.vis = const_cast<void *>(vis);
+  pb.wgt = wgt;
+  pb.dirty = const_cast<void *>(dirty);
+  return pb;
+}
+
+int check_precision(int precision) {
+  if (precision != 0 && precision != 1) {
+    set_error("precision must be 0 (FP64) or 1 (FP32-mixed), got %d", precision);
+    return kErrArg;
+  }
+  return kOk;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// reference entry points (src/astro/cugridder.cu:587-613)
+// ---------------------------------------------------------------------------
+extern "C" int ms2dirty_1(int nrow, int nxdirty, int nydirty, double fov, double freq, double *uvw, curig_c128 *vis,
+                          curig_c128 *dirty, double epsilon, double sigma, int sign) {
+  return host_call(true, nrow, nxdirty, nydirty, fov, freq, uvw, vis, nullptr, dirty, epsilon, sigma, sign, true);
+}
+
+extern "C" int ms2dirty_2(int nrow, int nxdirty, int nydirty, double fov, double freq, double *uvw, curig_c128 *vis,
+                          double *wgt, curig_c128 *dirty, double epsilon, double sigma, int sign) {
+  return host_call(true, nrow, nxdirty, nydirty, fov, freq, uvw, vis, wgt, dirty, epsilon, sigma, sign, true);
+}
+
+extern "C" int dirty2ms_1(int nrow, int nxdirty, int nydirty, double fov, double freq, double *uvw, curig_c128 *vis,
+                          curig_c128 *dirty, double epsilon, double sigma, int sign) {
+  return host_call(false, nrow, nxdirty, nydirty, fov, freq, uvw, vis, nullptr, dirty, epsilon, sigma, sign, true);
+}
+
+extern "C" int dirty2ms_2(int nrow, int nxdirty, int nydirty, double fov, double freq, double *uvw, curig_c128 *vis,
+                          double *wgt, curig_c128 *dirty, double epsilon, double sigma, int sign) {
+  return host_call(false, nrow, nxdirty, nydirty, fov, freq, uvw, vis, wgt, dirty, epsilon, sigma, sign, true);
+}
+
+// ---------------------------------------------------------------------------
+// additive exports
+// ---------------------------------------------------------------------------
+extern "C" int ms2dirty_f32(int nrow, int nxdirty, int nydirty, double fov, double freq, const double *uvw,
+                            const void *vis, const float *wgt, void *dirty, double epsilon, double sigma, int sign) {
+  return host_call(true, nrow, nxdirty, nydirty, fov, freq, uvw, const_cast<void *>(vis), wgt, dirty, epsilon, sigma,
+                   sign, false);
+}
+
+extern "C" int dirty2ms_f32(int nrow, int nxdirty, int nydirty, double fov, double freq, const double *uvw, void *vis,
+                            const float *wgt, const void *dirty, double epsilon, double sigma, int sign) {
+  return host_call(false, nrow, nxdirty, nydirty, fov, freq, uvw, vis, wgt, const_cast<void *>(dirty), epsilon, sigma,
+                   sign, false);
+}
+
+extern "C" int curig_ms2dirty_dev(int nrow, int nxdirty, int nydirty, double fov, double freq, const double *d_uvw,
+                                  const void *d_vis, const void *d_wgt, void *d_dirty, double epsilon, double sigma,
+                                  int sign, int precision, void *stream) {
+  CURIG_TRY(check_precision(precision));
+  CURIG_TRY(precheck(nrow, nxdirty, nydirty, epsilon, sigma));
+  Problem pb = dev_problem(nrow, nxdirty, nydirty, fov, freq, d_uvw, d_vis, d_wgt, d_dirty, epsilon, sigma, sign,
+                           precision);
+  return ms2dirty_device(pb, (hipStream_t)stream, &g_tuning, stats_ptr());
+}
+
+extern "C" int curig_dirty2ms_dev(int nrow, int nxdirty, int nydirty, double fov, double freq, const double *d_uvw,
+                                  void *d_vis, const void *d_wgt, const void *d_dirty, double epsilon, double sigma,
+                                  int sign, int precision, void *stream) {
+  CURIG_TRY(check_precision(precision));
+  CURIG_TRY(precheck(nrow, nxdirty, nydirty, epsilon, sigma));
+  Problem pb = dev_problem(nrow, nxdirty, nydirty, fov, freq, d_uvw, d_vis, d_wgt, d_dirty, epsilon, sigma, sign,
+                           precision);
+  return dirty2ms_device(pb, (hipStream_t)stream, &g_tuning, stats_ptr());
+}
+
+static int multi(bool to_image, int nrow, int nchan, int nx, int ny, double fov, const double *freq,
+                 const double *d_uvw, const void *d_vis, const void *d_wgt, const uint8_t *d_flag, const void *d_dirty,
+                 int per_channel, double eps, double sigma, int sign, int precision, void *stream) {
+  CURIG_TRY(check_precision(precision));
+  CURIG_TRY(precheck(nrow, nx, ny, eps, sigma));
+  if (nchan <= 0 || !freq) {
+    set_error("nchan must be > 0 with a frequency array");
+    return kErrArg;
+  }
+  const size_t cs = precision == 0 ? 16 : 8, rs = precision == 0 ? 8 : 4;
+  Stats total;
+  for (int c = 0; c < nchan; ++c) {
+    Problem pb = dev_problem(nrow, nx, ny, fov, freq[c], d_uvw, (const char *)d_vis + cs * c,
+                             d_wgt ? (const char *)d_wgt + rs * c : nullptr, d_dirty, eps, sigma, sign, precision);
+    pb.vis_stride = nchan;
+    pb.wgt_stride = nchan;
+    pb.flag = d_flag ? d_flag + c : nullptr;
+    pb.flag_stride = nchan;
+    if (per_channel) pb.dirty = (char *)const_cast<void *>(d_dirty) + cs * (size_t)nx * ny * c;
+    pb.accumulate = to_image && !per_channel && c > 0;
+    int ier = to_image ? ms2dirty_device(pb, (hipStream_t)stream, &g_tuning, stats_ptr())
+                       : dirty2ms_device(pb, (hipStream_t)stream, &g_tuning, stats_ptr());
+    if (ier != kOk) return ier;
+    if (g_stats_on) {
+      total.nvis += g_last_stats.nvis;
+      total.t_prep += g_last_stats.t_prep;
+      total.t_grid += g_last_stats.t_grid;
+      total.t_fft += g_last_stats.t_fft;
+      total.t_plane += g_last_stats.t_plane;
+      total.t_final += g_last_stats.t_final;
+      total.t_total += g_last_stats.t_total;
+      total.P += g_last_stats.P;
+    }
+  }
+  if (g_stats_on && nchan > 1) {
+    Stats last = g_last_stats;
+    g_last_stats = total;
+    g_last_stats.kw = last.kw;
+    g_last_stats.nf1 = last.nf1;
+    g_last_stats.nf2 = last.nf2;
+    g_last_stats.Pb = last.Pb;
+    g_last_stats.nbatch = last.nbatch;
+    g_last_stats.T1 = last.T1;
+    g_last_stats.T2 = last.T2;
+    g_last_stats.timed = 1;
+  }
+  return kOk;
+}
+
+extern "C" int curig_ms2dirty_multi_dev(int nrow, int nchan, int nxdirty, int nydirty, double fov, const double *freq,
+                                        const double *d_uvw, const void *d_vis, const void *d_wgt,
+                                        const uint8_t *d_flag, void *d_dirty, int per_channel, double epsilon,
+                                        double sigma, int sign, int precision, void *stream) {
+  return multi(true, nrow, nchan, nxdirty, nydirty, fov, freq, d_uvw, d_vis, d_wgt, d_flag, d_dirty, per_channel,
+               epsilon, sigma, sign, precision, stream);
+}
+
+extern "C" int curig_dirty2ms_multi_dev(int nrow, int nchan, int nxdirty, int nydirty, double fov, const double *freq,
+                                        const double *d_uvw, void *d_vis, const void *d_wgt, const uint8_t *d_flag,
+                                        const void *d_dirty, int per_channel, double epsilon, double sigma, int sign,
+                                        int precision, void *stream) {
+  return multi(false, nrow, nchan, nxdirty, nydirty, fov, freq, d_uvw, d_vis, d_wgt, d_flag, d_dirty, per_channel,
+               epsilon, sigma, sign, precision, stream);
+}
+
+extern "C" int curig_explicit_ms2dirty_dev(int nrow, int nxdirty, int nydirty, double fov, double freq,
+                                           const double *d_uvw, const void *d_vis, const void *d_wgt, int precision,
+                                           long npix, const int64_t *d_pix, void *d_out, int sign, void *stream) {
+  CURIG_TRY(check_precision(precision));
+  if (!d_out) {
+    set_error("NULL output");
+    return kErrArg;
+  }
+  Problem pb = dev_problem(nrow, nxdirty, nydirty, fov, freq, d_uvw, d_vis, d_wgt, d_out, 1e-6, 2.0, sign, precision);
+  return explicit_ms2dirty_device(pb, npix, d_pix, d_out, (hipStream_t)stream);
+}
+
+extern "C" const char *curig_last_error(void) { return last_error(); }
+
+extern "C" int curig_plan_info(int nrow_valid, int nxdirty, int nydirty, double fov, double freq, double epsilon,
+                               double sigma, int sign, double wmin, double wmax, int precision, int *ints,
+                               double *reals) {
+  CURIG_TRY(check_precision(precision));
+  if (nxdirty <= 0 || nydirty <= 0) {
+    set_error("image size %d x %d invalid", nxdirty, nydirty);
+    return kErrSize;
+  }
+  Geom g;
+  KernelOpts ko;
+  CURIG_TRY(plan_geometry(nrow_valid, nxdirty, nydirty, fov, freq, epsilon, sigma, sign, wmin, wmax, precision == 0,
+                          &g_tuning, &g, &ko));
+  if (ints) {
+    int v[8] = {g.kw, g.nf1, g.nf2, g.P, g.Pb, g.nbatch, g.T1, g.T2};
+    memcpy(ints, v, sizeof(v));
+  }
+  if (reals) {
+    double r[8] = {ko.beta, ko.c, g.gamma, g.h, g.C, g.D, g.dx, g.dy};
+    memcpy(reals, r, sizeof(r));
+  }
+  return kOk;
+}
+
+extern "C" int curig_kernel_fseries(double epsilon, double sigma, int nf, double *out) {
+  KernelOpts ko;
+  int ier = setup_conv_opts(ko, epsilon, sigma);
+  if (ier == kErrSigma) return ier;
+  if (nf < 2 || !out) {
+    set_error("nf must be >= 2");
+    return kErrArg;
+  }
+  fseries_grid(ko, nf, out);
+  return ier;
+}
+
+extern "C" int curig_gauss_legendre(int n, double *x, double *w) {
+  if (n < 1 || !x || !w) {
+    set_error("n must be >= 1");
+    return kErrArg;
+  }
+  gauss_legendre(n, x, w);
+  return kOk;
+}
+
+extern "C" int curig_set_tuning(int tile_x, int tile_y, int planes_per_batch) {
+  if (tile_x < 0 || tile_y < 0 || planes_per_batch < 0 || tile_x > 256 || tile_y > 256 ||
+      (size_t)tile_x * tile_y > 4096) {
+    set_error("invalid tuning %d x %d, %d planes", tile_x, tile_y, planes_per_batch);
+    return kErrArg;
+  }
+  g_tuning.T1 = tile_x;
+  g_tuning.T2 = tile_y;
+  g_tuning.Pb = planes_per_batch;
+  return kOk;
+}
+
+extern "C" int curig_enable_stats(int on) {
+  g_stats_on = on != 0;
+  return kOk;
+}
+
+extern "C" int curig_get_stats(int *ints, double *reals) {
+  const Stats &s = g_last_stats;
+  if (ints) {
+    int v[8] = {s.kw, s.nf1, s.nf2, s.P, s.Pb, s.nbatch, s.T1, s.T2};
+    memcpy(ints, v, sizeof(v));
+  }
+  if (reals) {
+    double r[8] = {(double)s.nvis, s.t_prep, s.t_grid, s.t_fft, s.t_plane, s.t_final, s.t_total, (double)s.timed};
+    memcpy(reals, r, sizeof(r));
+  }
+  return kOk;
+}
+
+extern "C" void curig_release(void) {
+  std::lock_guard<std::mutex> lk(g_host_mu);
+  for (auto &h : g_host) {
+    for (int i = 0; i < 4; ++i)
+      if (h.buf[i]) (void)hipFree(h.buf[i]);
+    if (h.stream) (void)hipStreamDestroy(h.stream);
+  }
+  g_host.clear();
+  release_workspace();
+}
+
+// ---------------------------------------------------------------------------
+// C++ plan API (src/astro/cugridder.cu:105-307)
+// ---------------------------------------------------------------------------
+__global__ void pack_uvw_kernel(const double *u, const double *v, const double *w, double *uvw, int n, double s) {
+  for (long j = blockIdx.x * (long)blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) {
+    uvw[3 * j] = u[j] * s;
+    uvw[3 * j + 1] = v[j] * s;
+    uvw[3 * j + 2] = w[j] * s;
+  }
+}
+
+int gridder_setting(int N1, int N2, int method, int kerevalmeth, int w_term_method, PCS tol, int direction,
+                    double sigma, int iflag, int batchsize, int M, int channel, PCS fov, visibility *pointer_v,
+                    PCS *d_u, PCS *d_v, PCS *d_w, CUCPX *d_c, curafft_plan *plan, ragridder_plan *gridder_plan) {
+  (void)iflag;
+  (void)batchsize;
+  if (!plan || !gridder_plan || !pointer_v || !pointer_v->frequency) {
+    set_error("NULL plan or visibility struct");
+    return kErrArg;
+  }
+  if (w_term_method != 1 || method != 0 || kerevalmeth != 0) {
+    set_error("only improved w-stacking (w_term_method 1), method 0, kerevalmeth 0 are provided");
+    return kErrArg;
+  }
+  KernelOpts ko;
+  int ier = setup_conv_opts(ko, tol, sigma);
+  if (ier != kOk) return ier;
+  if (M < 0 || N1 <= 0 || N2 <= 0) {
+    set_error("invalid sizes");
+    return M < 0 ? kErrNrow : kErrSize;
+  }
+  plan->tol = tol;
+  plan->copts.kw = ko.kw;
+  plan->copts.direction = direction;
+  plan->copts.pirange = 1;
+  plan->copts.upsampfac = sigma;
+  plan->copts.ES_beta = ko.beta;
+  plan->copts.ES_halfwidth = ko.halfwidth;
+  plan->copts.ES_c = ko.c;
+  plan->opts.upsampfac = sigma;
+  plan->iflag = direction > 0 ? 1 : -1;
+  plan->type = direction > 0 ? 1 : 2;
+  plan->dim = 3;
+  plan->mode_flag = 1;
+  plan->M = M;
+  plan->ms = N1;
+  plan->mt = N2;
+  plan->mu = 1;
+  plan->execute_flow = 1;
+  plan->d_u = d_u;
+  plan->d_v = d_v;
+  plan->d_w = d_w;
+  plan->d_c = d_c;
+  gridder_plan->kv = *pointer_v;
+  gridder_plan->fov = fov;
+  gridder_plan->width = N1;
+  gridder_plan->height = N2;
+  gridder_plan->nrow = M;
+  gridder_plan->channel = channel;
+  gridder_plan->w_term_method = w_term_method;
+  gridder_plan->speedoflight = (int)kSpeedOfLight;
+  gridder_plan->pixelsize_x = fov / 180.0 * M_PI / (double)N2;
+  gridder_plan->pixelsize_y = fov / 180.0 * M_PI / (double)N1;
+  // pack u, v, w (metres; pirange = 1 means already multiplied by 2 pi)
+  double s = pointer_v->pirange ? 1.0 / (2 * M_PI) : 1.0;
+  CURIG_HIP(hipMalloc(&plan->d_uvw, sizeof(double) * 3 * (size_t)std::max(M, 1)));
+  if (M > 0) {
+    pack_uvw_kernel<<<(M + 255) / 256, 256>>>(d_u, d_v, d_w, plan->d_uvw, M, s);
+    CURIG_HIP(hipGetLastError());
+  }
+  if (pointer_v->weight && M > 0) {
+    CURIG_HIP(hipMalloc(&plan->d_wgt, sizeof(double) * M));
+    CURIG_HIP(hipMemcpy(plan->d_wgt, pointer_v->weight, sizeof(double) * M, hipMemcpyHostToDevice));
+  }
+  // plan sizes for the caller (w range from the host copy when present)
+  double wmin = 0, wmax = 0;
+  if (M > 0) {
+    std::vector<double> hw(M);
+    CURIG_HIP(hipMemcpy(hw.data(), d_w, sizeof(double) * M, hipMemcpyDeviceToHost));
+    wmin = wmax = hw[0] * s;
+    for (double x : hw) {
+      wmin = std::min(wmin, x * s);
+      wmax = std::max(wmax, x * s);
+    }
+  }
+  Geom g;
+  ier = plan_geometry(M, N2, N1, fov, pointer_v->frequency[0], tol, sigma, pointer_v->sign, wmin, wmax,
+                      plan->opts.precision == 0, &g_tuning, &g, nullptr);
+  if (ier != kOk) return ier;
+  plan->nf1 = g.nf1;
+  plan->nf2 = g.nf2;
+  plan->nf3 = g.P;
+  plan->batchsize = g.Pb;
+  plan->ta.i_center[0] = g.C;
+  plan->ta.o_center[0] = g.D;
+  plan->ta.gamma[0] = g.gamma;
+  plan->ta.h[0] = g.h;
+  gridder_plan->num_w = g.P;
+  return kOk;
+}
+
+int gridder_execution(curafft_plan *plan, ragridder_plan *gridder_plan) {
+  if (!plan || !gridder_plan || !plan->fk || (plan->M > 0 && !plan->d_c)) {
+    set_error("plan->fk (image) and plan->d_c (visibilities) must be set");
+    return kErrArg;
+  }
+  Problem pb;
+  pb.nrow = plan->M;
+  pb.nx = plan->mt;
+  pb.ny = plan->ms;
+  pb.fov = gridder_plan->fov;
+  pb.freq = gridder_plan->kv.frequency[0];
+  pb.sigma = plan->copts.upsampfac;
+  pb.sign = gridder_plan->kv.sign;
+  pb.dbl = true;
+  pb.uvw = plan->d_uvw;
+  pb.vis = plan->d_c;
+  pb.wgt = plan->d_wgt;
+  pb.dirty = plan->fk;
+  pb.eps = plan->tol;
+  hipStream_t st = plan->streams ? plan->streams[0] : nullptr;
+  int ier = plan->copts.direction == 1 ? ms2dirty_device(pb, st, &g_tuning, stats_ptr())
+                                       : dirty2ms_device(pb, st, &g_tuning, stats_ptr());
+  if (ier != kOk) return ier;
+  CURIG_HIP(hipStreamSynchronize(st));
+  return kOk;
+}
+
+int gridder_destroy(curafft_plan *plan, ragridder_plan *gridder_plan) {
+  if (plan) {
+    if (plan->d_uvw) (void)hipFree(plan->d_uvw);
+    if (plan->d_wgt) (void)hipFree(plan->d_wgt);
+    if (plan->d_u) (void)hipFree(plan->d_u);
+    if (plan->d_v) (void)hipFree(plan->d_v);
+    if (plan->d_w) (void)hipFree(plan->d_w);
+    if (plan->d_c) (void)hipFree(plan->d_c);
+    if (plan->fk) (void)hipFree(plan->fk);
+    free(plan);
+  }
+  if (gridder_plan) {
+    free(gridder_plan->dirty_image);
+    free(gridder_plan->kv.u);
+    free(gridder_plan->kv.v);
+    free(gridder_plan->kv.w);
+    free(gridder_plan->kv.vis);
+    free(gridder_plan->kv.frequency);
+    free(gridder_plan->kv.weight);
+    free(gridder_plan);
+  }
+  return kOk;
+}

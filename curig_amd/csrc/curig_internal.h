@@ -1,0 +1,94 @@
+// Internal declarations shared by the gridder's host and device sources.
+//
+// Layout conventions (fixed by the reference, src/astro/cugridder.cu:188,399):
+//   image  dirty[nx][ny], x-major; N1 = ny (fast axis, pairs with v), N2 = nx (pairs with u)
+//   grid   plane[P][nf2][nf1]; the grid's fast axis (nf1) carries v, the slow axis (nf2) carries u
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+namespace curig {
+
+// Return codes of every entry point (0 = success).  1 and 2 keep the meaning
+// setup_conv_opts gives them in the reference (conv_interp_invoker.cu:46-116).
+enum Status : int {
+  kOk = 0,
+  kErrEpsTooSmall = 1,  // epsilon < 1.1e-16
+  kErrSigma = 2,        // upsampling factor <= 1
+  kErrSize = 3,         // image size <= 0 or grid too large
+  kErrNrow = 4,         // nrow < 0
+  kErrArg = 5,          // NULL pointer, sign not +-1, bad option
+  kErrHip = 6,          // HIP runtime error
+  kErrAlloc = 7,        // device allocation failed
+  kErrFft = 8,          // rocFFT error
+  kErrNonFinite = 9,    // non-finite uvw / frequency
+  kErrScratch = 10,     // request exceeds a fixed-size buffer
+};
+
+void set_error(const char *fmt, ...);
+const char *last_error();
+
+#define CURIG_HIP(call)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      ::curig::set_error("%s:%d %s -> %s", __FILE__, __LINE__, #call,                \
+                         hipGetErrorString(e_));                                     \
+      return (e_ == hipErrorOutOfMemory) ? ::curig::kErrAlloc : ::curig::kErrHip;     \
+    }                                                                                \
+  } while (0)
+
+#define CURIG_TRY(expr)        \
+  do {                         \
+    int s_ = (expr);           \
+    if (s_ != 0) return s_;    \
+  } while (0)
+
+constexpr double kSpeedOfLight = 299792458.0;  // include/utils.h:70
+constexpr double kEpsilonMin = 1.1e-16;        // include/utils.h:66
+constexpr int kMaxKernelWidth = 28;            // include/curafft_plan.h:14
+constexpr int kMaxQuadNodes = 2 * (2 + 3 * kMaxKernelWidth / 2);  // 2p, deconv.cu:75
+
+// ES kernel options (conv_opts, include/curafft_plan.h:28-48)
+struct KernelOpts {
+  int kw = 0;
+  double beta = 0, c = 0, halfwidth = 0, upsampfac = 2.0;
+};
+
+// Host plan math (plan_math.cpp)
+int setup_conv_opts(KernelOpts &o, double eps, double upsampfac);
+int next235beven(int n, int b);
+int get_num_cells(int ms, const KernelOpts &o);
+void set_nhg_type3(double S, double X, const KernelOpts &o, int &nf, double &h, double &gam);
+void gauss_legendre(int n, double *x, double *w);
+// 2p scaled nodes x_n and weights g_n = alpha w_n phi_norm(x_n); returns 2p
+int kernel_quadrature(const KernelOpts &o, double *x, double *g);
+// phihat_norm(k) (-1)^k for k = 0..nf/2 (normalised kernel, peak 1)
+void fseries_grid(const KernelOpts &o, int nf, double *out);
+
+// Everything the device kernels need for one (channel, direction) problem.
+struct Geom {
+  int N1, N2;         // image: N1 = ny (fast), N2 = nx
+  int nf1, nf2, P;    // grid sizes: nf1 (v), nf2 (u), w planes
+  int kw;
+  double beta, c;     // normalised ES kernel exp(beta (sqrt(1 - c d^2) - 1))
+  double dx, dy;      // pixel sizes (rad): dx pairs with u / image rows, dy with v / columns
+  double C, D, gamma, h;  // type-3 w parameters (t3attrib): i_center, o_center, gamma, h
+  double uscale, vscale, wscale;  // 2 pi f/c dx, 2 pi f/c dy, 2 pi f/c sign
+  int T1, T2, ntx, nty, ntiles;   // tile size (cells) along nf1 / nf2, tile counts
+  int Pb, nbatch;     // planes per batch, number of batches
+  int nquad;          // number of quadrature nodes for phihat3
+};
+
+// Tunables (autotuned per problem class; defaults are safe for any size)
+struct Tuning {
+  int T1 = 0, T2 = 0;   // 0 -> default for the precision
+  int Pb = 0;           // 0 -> from the memory budget
+  size_t grid_budget_bytes = size_t(32) << 30;
+};
+
+}  // namespace curig

@@ -1,0 +1,52 @@
+// Device-resident gridder core: one channel, one direction, all pointers on the GPU.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "curig_internal.h"
+
+namespace curig {
+
+struct Problem {
+  int nrow = 0, nx = 0, ny = 0;
+  double fov = 0, freq = 0, eps = 1e-6, sigma = 2.0;
+  int sign = -1;
+  bool dbl = true;              // FP64 path, or FP32-mixed (complex64 data, FP32 grid)
+  const double *uvw = nullptr;  // [nrow][3] metres, FP64
+  void *vis = nullptr;          // complex (double2 / float2); row j at vis[j * vis_stride]
+  long vis_stride = 1;
+  const void *wgt = nullptr;    // real (double / float) or nullptr; row j at wgt[j * wgt_stride]
+  long wgt_stride = 1;
+  const uint8_t *flag = nullptr;  // nonzero = row excluded; row j at flag[j * flag_stride]
+  long flag_stride = 1;
+  void *dirty = nullptr;        // complex [nx][ny]
+  bool accumulate = false;      // ms2dirty adds into dirty instead of overwriting it
+};
+
+// Filled on request: plan sizes and per-stage device times (ms, hipEvent).
+struct Stats {
+  int kw = 0, nf1 = 0, nf2 = 0, P = 0, Pb = 0, nbatch = 0, T1 = 0, T2 = 0;
+  long nvis = 0;               // unflagged rows gridded
+  double beta = 0, gamma = 0, C = 0, D = 0;
+  double t_prep = 0, t_grid = 0, t_fft = 0, t_plane = 0, t_final = 0, t_total = 0;
+  // t_grid: spread (ms2dirty) or interp (dirty2ms); t_plane: w-accumulate / w-fill
+  int timed = 0;               // 1 when the times above were measured
+};
+
+int ms2dirty_device(const Problem &pb, hipStream_t stream, const Tuning *tune, Stats *st);
+int dirty2ms_device(const Problem &pb, hipStream_t stream, const Tuning *tune, Stats *st);
+
+// FP64 direct sum on the GPU (the reference's explicit_gridder, precomp.cu:242-275),
+// at npix chosen pixels (pix = i*ny + k) or, with pix == nullptr, at every pixel.
+int explicit_ms2dirty_device(const Problem &pb, long npix, const int64_t *pix, void *out,
+                             hipStream_t stream);
+
+// Host-only plan computation used by the tests (no device access).
+int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, double eps, double sigma,
+                  int sign, double wmin_m, double wmax_m, bool dbl, const Tuning *tune, Geom *g,
+                  KernelOpts *ko);
+
+void release_workspace();
+
+}  // namespace curig

@@ -1,0 +1,1230 @@
+// ES-kernel improved-w-stacking gridder for CDNA4 (gfx950).
+//
+// Pipeline (one channel):
+//   ms2dirty  prep: w range -> plan; bin rows by (w-plane batch, grid tile of the
+//             kernel's first cell); counting sort into 32/56-byte records
+//             for each batch of Pb w planes:
+//               spread   tile-owner gather into LDS, every grid cell written once
+//               rocFFT   batched 2-D, sign +1
+//               w-accum  acc[pix] += sum_p plane_p[cell(pix)] e^{+i z'(p - P/2)} (FP64)
+//             epilogue: ES / w corrections, / n
+//   dirty2ms  prologue: corrections, / n; for each batch: w-fill (zero-padded planes
+//             with the w phase), rocFFT sign -1, interpolation into FP64 per-row
+//             accumulators; finalize: weight and e^{-i D w}, scatter to row order.
+//
+// Mathematically this is the reference's operator (src/astro/ra_exec.cu:141-436):
+// same grid sizes, w-plane count and scaling (set_nhg_type3), kernel and
+// correction factors; the full nf3 x nf2 x nf1 grid of the reference is never
+// materialised (only Pb planes at a time), and the kernel is normalised to peak 1
+// (its Fourier-series factors carry the same normalisation, so it cancels).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "curig_internal.h"
+#include "fft.h"
+#include "gridder.h"
+
+namespace curig {
+
+#define M_1_2PI_D 0.159154943091895336  // include/utils.h:61
+#define M_2PI_D 6.28318530717958648     // include/utils.h:62
+
+// --------------------------------------------------------------------------
+// types
+// --------------------------------------------------------------------------
+template <typename T> struct V2;
+template <> struct V2<float> { using type = float2; };
+template <> struct V2<double> { using type = double2; };
+template <typename T> using v2_t = typename V2<T>::type;
+
+// Sorted visibility record: first grid cell of the kernel support on each axis
+// (already reduced modulo the axis length), offset of that first cell from the
+// exact position (kernel argument of the first tap), and the weighted value.
+template <typename T> struct Rec;
+template <> struct __align__(16) Rec<float> {
+  int ix, iy, iw;
+  float dx, dy, dw, re, im;
+};
+template <> struct __align__(8) Rec<double> {
+  int ix, iy, iw, pad;
+  double dx, dy, dw, re, im;
+};
+
+// --------------------------------------------------------------------------
+// device helpers
+// --------------------------------------------------------------------------
+__device__ __forceinline__ float es_eval(float d, float beta, float c, float hw) {
+  float s = 1.0f - c * d * d;
+  return (fabsf(d) < hw && s > 0.0f) ? expf(beta * (sqrtf(s) - 1.0f)) : 0.0f;
+}
+__device__ __forceinline__ double es_eval(double d, double beta, double c, double hw) {
+  double s = 1.0 - c * d * d;
+  return (fabs(d) < hw && s > 0.0) ? exp(beta * (sqrt(s) - 1.0)) : 0.0;
+}
+
+__device__ __forceinline__ int imod(int a, int n) {
+  int r = a % n;
+  return r < 0 ? r + n : r;
+}
+
+// x in radians, folded into [-pi, pi) and rescaled to grid units in [0, nf):
+// SHIFT_RESCALE(x, nf, 1) (include/conv.h:21) after an exact 2pi fold.
+__device__ __forceinline__ double rescale_fold(double x, int nf) {
+  double xf = x - floor(x * M_1_2PI_D + 0.5) * M_2PI_D;
+  return (xf * M_1_2PI_D + 0.5) * nf;
+}
+
+// n - 1 without cancellation, and n, for pixel (row, col)
+__device__ __forceinline__ void pixel_n(int row, int col, const Geom &g, double &nm1, double &n) {
+  double l = (double)(row - g.N2 / 2) * g.dx;
+  double m = (double)(col - g.N1 / 2) * g.dy;
+  double r2 = l * l + m * m;
+  n = sqrt(1.0 - r2);
+  nm1 = -r2 / (n + 1.0);
+}
+
+struct Coords {
+  int ix, iy, iw;
+  double dx, dy, dw, we;
+};
+
+// Grid coordinates of row j (src/astro/precomp.cu:48-72 scaling, cuft.cu:53-75
+// type-3 pre-stage, conv.cu:176-198 start cell and first kernel argument).
+__device__ __forceinline__ Coords row_coords(const double *__restrict__ uvw, long j, const Geom &g) {
+  double u = uvw[3 * j], v = uvw[3 * j + 1], w = uvw[3 * j + 2];
+  Coords c;
+  double xu = u * g.uscale;  // grid slow axis (nf2)
+  double yv = v * g.vscale;  // grid fast axis (nf1)
+  c.we = w * g.wscale;
+  double wp = (c.we - g.C) / g.gamma;
+  double tx = rescale_fold(yv, g.nf1);
+  double ty = rescale_fold(xu, g.nf2);
+  double tw = (wp * M_1_2PI_D + 0.5) * g.P;
+  double hk = 0.5 * g.kw;
+  double sx = ceil(tx - hk), sy = ceil(ty - hk), sw = ceil(tw - hk);
+  c.dx = sx - tx;
+  c.dy = sy - ty;
+  c.dw = sw - tw;
+  c.ix = imod((int)sx, g.nf1);
+  c.iy = imod((int)sy, g.nf2);
+  c.iw = imod((int)sw, g.P);
+  return c;
+}
+
+__device__ __forceinline__ int row_bin(const Coords &c, const Geom &g) {
+  int tile = (c.iy / g.T2) * g.ntx + (c.ix / g.T1);
+  return (c.iw / g.Pb) * g.ntiles + tile;
+}
+
+template <typename T> __device__ __forceinline__ double2 load_c(const void *p, long j);
+template <> __device__ __forceinline__ double2 load_c<double>(const void *p, long j) {
+  return ((const double2 *)p)[j];
+}
+template <> __device__ __forceinline__ double2 load_c<float>(const void *p, long j) {
+  float2 v = ((const float2 *)p)[j];
+  return make_double2(v.x, v.y);
+}
+template <typename T> __device__ __forceinline__ double load_r(const void *p, long j) {
+  return (double)((const T *)p)[j];
+}
+template <typename T> __device__ __forceinline__ void store_c(void *p, long j, double re, double im);
+template <> __device__ __forceinline__ void store_c<double>(void *p, long j, double re, double im) {
+  ((double2 *)p)[j] = make_double2(re, im);
+}
+template <> __device__ __forceinline__ void store_c<float>(void *p, long j, double re, double im) {
+  ((float2 *)p)[j] = make_float2((float)re, (float)im);
+}
+template <typename T> __device__ __forceinline__ void add_c(void *p, long j, double re, double im);
+template <> __device__ __forceinline__ void add_c<double>(void *p, long j, double re, double im) {
+  double2 o = ((double2 *)p)[j];
+  ((double2 *)p)[j] = make_double2(o.x + re, o.y + im);
+}
+template <> __device__ __forceinline__ void add_c<float>(void *p, long j, double re, double im) {
+  float2 o = ((float2 *)p)[j];
+  ((float2 *)p)[j] = make_float2((float)(o.x + re), (float)(o.y + im));
+}
+
+__device__ __forceinline__ bool row_skipped(const uint8_t *flag, long fs, long j) {
+  return flag != nullptr && flag[j * fs] != 0;
+}
+
+// --------------------------------------------------------------------------
+// prep kernels
+// --------------------------------------------------------------------------
+// Per-block min / max of w (metres) over unflagged rows.
+__global__ void __launch_bounds__(256) w_minmax_kernel(const double *__restrict__ uvw, int nrow,
+                                                        const uint8_t *__restrict__ flag, long fs,
+                                                        double *__restrict__ part, int *__restrict__ cnt) {
+  __shared__ double smin[256], smax[256];
+  __shared__ int scnt[256];
+  double lo = INFINITY, hi = -INFINITY;
+  int n = 0;
+  for (long j = blockIdx.x * (long)blockDim.x + threadIdx.x; j < nrow; j += (long)gridDim.x * blockDim.x) {
+    if (row_skipped(flag, fs, j)) continue;
+    double w = uvw[3 * j + 2];
+    // NaN propagates to the host check through the sum of |u|+|v|+|w|
+    double a = uvw[3 * j] + uvw[3 * j + 1];
+    if (!(fabs(a) < INFINITY) || !(fabs(w) < INFINITY)) {
+      lo = NAN;
+      hi = NAN;
+    } else if (lo == lo) {
+      lo = fmin(lo, w);
+      hi = fmax(hi, w);
+    }
+    ++n;
+  }
+  smin[threadIdx.x] = lo;
+  smax[threadIdx.x] = hi;
+  scnt[threadIdx.x] = n;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      double a = smin[threadIdx.x], b = smin[threadIdx.x + s];
+      smin[threadIdx.x] = (a != a || b != b) ? NAN : fmin(a, b);
+      a = smax[threadIdx.x];
+      b = smax[threadIdx.x + s];
+      smax[threadIdx.x] = (a != a || b != b) ? NAN : fmax(a, b);
+      scnt[threadIdx.x] += scnt[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = smin[0];
+    part[2 * blockIdx.x + 1] = smax[0];
+    cnt[blockIdx.x] = scnt[0];
+  }
+}
+
+__global__ void fill_int_kernel(int *__restrict__ p, long n, int v) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void fill_zero_c128_kernel(double2 *__restrict__ p, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    p[i] = make_double2(0.0, 0.0);
+}
+
+template <typename T>
+__global__ void zero_rows_kernel(void *__restrict__ vis, long stride, int nrow) {
+  for (long j = blockIdx.x * (long)blockDim.x + threadIdx.x; j < nrow; j += (long)gridDim.x * blockDim.x)
+    store_c<T>(vis, j * stride, 0.0, 0.0);
+}
+
+// Count rows per bin; each row keeps its rank inside the bin (GM-sort idea).
+__global__ void __launch_bounds__(256) bin_count_kernel(const double *__restrict__ uvw, int nrow,
+                                                         const uint8_t *__restrict__ flag, long fs, Geom g,
+                                                         int *__restrict__ hist, int2 *__restrict__ binrank) {
+  for (long j = blockIdx.x * (long)blockDim.x + threadIdx.x; j < nrow; j += (long)gridDim.x * blockDim.x) {
+    if (row_skipped(flag, fs, j)) {
+      binrank[j] = make_int2(-1, 0);
+      continue;
+    }
+    Coords c = row_coords(uvw, j, g);
+    int b = row_bin(c, g);
+    int r = atomicAdd(&hist[b], 1);
+    binrank[j] = make_int2(b, r);
+  }
+}
+
+// Exclusive scan of n ints: three passes (block sums, scan of sums, apply).
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+__global__ void __launch_bounds__(kScanBlock) scan_reduce_kernel(const int *__restrict__ in, long n,
+                                                                  int *__restrict__ bsum) {
+  __shared__ int s[kScanBlock];
+  long base = (long)blockIdx.x * kScanTile;
+  int acc = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    long i = base + (long)k * kScanBlock + threadIdx.x;
+    if (i < n) acc += in[i];
+  }
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = kScanBlock / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = s[0];
+}
+
+// single block: exclusive scan of nb block sums in place
+__global__ void __launch_bounds__(1024) scan_sums_kernel(int *__restrict__ bsum, int nb) {
+  __shared__ int s[1024];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nb; base += 1024) {
+    int i = base + threadIdx.x;
+    int v = i < nb ? bsum[i] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      int t = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nb) bsum[i] = carry + s[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += s[1023];
+    __syncthreads();
+  }
+}
+
+// out[i] = bsum[block] + exclusive scan within the tile; out[n] = total
+__global__ void __launch_bounds__(kScanBlock) scan_apply_kernel(const int *__restrict__ in, long n,
+                                                                 const int *__restrict__ bsum,
+                                                                 int *__restrict__ out, int total) {
+  __shared__ int s[kScanBlock];
+  long base = (long)blockIdx.x * kScanTile + (long)threadIdx.x * kScanItems;
+  int loc[kScanItems];
+  int sum = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    long i = base + k;
+    loc[k] = i < n ? in[i] : 0;
+    sum += loc[k];
+  }
+  s[threadIdx.x] = sum;
+  __syncthreads();
+  for (int off = 1; off < kScanBlock; off <<= 1) {
+    int t = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int run = bsum[blockIdx.x] + s[threadIdx.x] - sum;
+  for (int k = 0; k < kScanItems; ++k) {
+    long i = base + k;
+    if (i < n) out[i] = run;
+    run += loc[k];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = total;
+}
+
+// Scatter rows into bin order; the value carries the weight and e^{+i D w}
+// (ms2dirty pre-stages, src/astro/ra_exec.cu:98-110).
+template <typename T>
+__global__ void __launch_bounds__(256) bin_scatter_kernel(const double *__restrict__ uvw, int nrow,
+                                                           const void *__restrict__ vis, long vs,
+                                                           const void *__restrict__ wgt, long ws, Geom g,
+                                                           const int2 *__restrict__ binrank,
+                                                           const int *__restrict__ off, Rec<T> *__restrict__ recs,
+                                                           int *__restrict__ perm, int with_value) {
+  for (long j = blockIdx.x * (long)blockDim.x + threadIdx.x; j < nrow; j += (long)gridDim.x * blockDim.x) {
+    int2 br = binrank[j];
+    if (br.x < 0) continue;
+    Coords c = row_coords(uvw, j, g);
+    int dst = off[br.x] + br.y;
+    Rec<T> r;
+    r.ix = c.ix;
+    r.iy = c.iy;
+    r.iw = c.iw;
+    r.dx = (T)c.dx;
+    r.dy = (T)c.dy;
+    r.dw = (T)c.dw;
+    if (with_value) {
+      double2 v = load_c<T>(vis, j * vs);
+      double wt = wgt ? load_r<T>(wgt, j * ws) : 1.0;
+      double s, co;
+      sincos(g.D * c.we, &s, &co);
+      r.re = (T)(wt * (v.x * co - v.y * s));
+      r.im = (T)(wt * (v.x * s + v.y * co));
+    } else {
+      r.re = 0;
+      r.im = 0;
+    }
+    recs[dst] = r;
+    perm[dst] = (int)j;
+  }
+}
+
+// --------------------------------------------------------------------------
+// spread: one block owns one T1 x T2 tile of one w plane and writes every cell
+// of it exactly once (no global atomics, no memset of the grid).
+// --------------------------------------------------------------------------
+// Walk the cyclic interval [a, a+len) of an axis of length n cut into tiles of
+// size t (last tile may be partial), calling f(tile) once per tile touched.
+template <typename F>
+__device__ __forceinline__ void walk_tiles(int a, int len, int n, int t, int ntile, F f) {
+  if (len > n) len = n;
+  int tile = a / t;
+  int end = min((tile + 1) * t, n);
+  int covered = end - a;
+  f(tile);
+  int visited = 1;
+  while (covered < len && visited < ntile) {
+    tile = (tile + 1 == ntile) ? 0 : tile + 1;
+    covered += min((tile + 1) * t, n) - tile * t;
+    f(tile);
+    ++visited;
+  }
+}
+
+template <typename T, int KWM>
+__global__ void __launch_bounds__(256) spread_tile_kernel(const Rec<T> *__restrict__ recs,
+                                                          const int *__restrict__ off,
+                                                          v2_t<T> *__restrict__ grid, Geom g, int p0) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *acc = reinterpret_cast<T *>(smem_raw);
+  const int tile = blockIdx.x;
+  const int pl = blockIdx.y;
+  const int p = p0 + pl;
+  const int tx = tile % g.ntx, ty = tile / g.ntx;
+  const int x0 = tx * g.T1, y0 = ty * g.T2;
+  const int wx = min(g.T1, g.nf1 - x0), wy = min(g.T2, g.nf2 - y0);
+  const int ncell = g.T1 * g.T2;
+  for (int i = threadIdx.x; i < 2 * ncell; i += blockDim.x) acc[i] = (T)0;
+  __syncthreads();
+
+  const int kw = g.kw;
+  const T beta = (T)g.beta, cc = (T)g.c, hw = (T)(0.5 * kw);
+  // source rows: first plane in [p-kw+1, p], first cell in [x0-kw+1, x0+wx-1] x [y0-kw+1, y0+wy-1]
+  const int nbat = g.nbatch;
+  walk_tiles(imod(p - kw + 1, g.P), kw, g.P, g.Pb, nbat, [&](int sb) {
+    walk_tiles(imod(y0 - kw + 1, g.nf2), wy + kw - 1, g.nf2, g.T2, g.nty, [&](int sty) {
+      walk_tiles(imod(x0 - kw + 1, g.nf1), wx + kw - 1, g.nf1, g.T1, g.ntx, [&](int stx) {
+        const int bin = sb * g.ntiles + sty * g.ntx + stx;
+        const int r0 = off[bin], r1 = off[bin + 1];
+        for (int r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+          const Rec<T> rc = recs[r];
+          int k = p - rc.iw;
+          if (k < 0) k += g.P;
+          if (k >= kw) continue;
+          int rx = rc.ix - x0;
+          if (rx < 0) rx += g.nf1;
+          int ry = rc.iy - y0;
+          if (ry < 0) ry += g.nf2;
+          // quick reject: footprint misses the tile on either axis
+          // (a footprint that wraps past the axis end always covers cell 0 of the tile row)
+          const bool hitx = (rx < wx) || (rx + kw - 1 >= g.nf1);
+          const bool hity = (ry < wy) || (ry + kw - 1 >= g.nf2);
+          if (!hitx || !hity) continue;
+          const T vw = es_eval(rc.dw + (T)k, beta, cc, hw);
+          const T vre = rc.re * vw, vim = rc.im * vw;
+          T kx[KWM];
+#pragma unroll
+          for (int i = 0; i < KWM; ++i) kx[i] = (i < kw) ? es_eval(rc.dx + (T)i, beta, cc, hw) : (T)0;
+#pragma unroll
+          for (int jj = 0; jj < KWM; ++jj) {
+            if (jj >= kw) break;
+            int cy = ry + jj;
+            if (cy >= g.nf2) cy -= g.nf2;
+            if (cy >= wy) continue;
+            const T ky = es_eval(rc.dy + (T)jj, beta, cc, hw);
+            const T are = vre * ky, aim = vim * ky;
+            T *rowp = acc + 2 * (cy * g.T1);
+#pragma unroll
+            for (int i = 0; i < KWM; ++i) {
+              if (i >= kw) break;
+              int cx = rx + i;
+              if (cx >= g.nf1) cx -= g.nf1;
+              if (cx >= wx) continue;
+              atomicAdd(rowp + 2 * cx, are * kx[i]);
+              atomicAdd(rowp + 2 * cx + 1, aim * kx[i]);
+            }
+          }
+        }
+      });
+    });
+  });
+  __syncthreads();
+  v2_t<T> *out = grid + ((size_t)pl * g.nf2 + y0) * g.nf1 + x0;
+  for (int i = threadIdx.x; i < wy * g.T1; i += blockDim.x) {
+    int cy = i / g.T1, cx = i - cy * g.T1;
+    if (cx < wx) {
+      v2_t<T> v;
+      v.x = acc[2 * i];
+      v.y = acc[2 * i + 1];
+      out[(size_t)cy * g.nf1 + cx] = v;
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// w-accumulate (ms2dirty): acc[pix] += sum_{p in batch} F_p[cell(pix)] e^{+i z'(p - P/2)}
+// src/fourier/cuft.cu:520-572 restated per batch, FP64 phase and sum.
+// --------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) wacc_kernel(const v2_t<T> *__restrict__ grid, double2 *__restrict__ acc,
+                                                    Geom g, int p0, int nb, int first) {
+  const long npix = (long)g.N1 * g.N2;
+  const size_t plane = (size_t)g.nf1 * g.nf2;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < npix; idx += (long)gridDim.x * blockDim.x) {
+    int row = (int)(idx / g.N1), col = (int)(idx - (long)row * g.N1);
+    int w1 = col >= g.N1 / 2 ? col - g.N1 / 2 : g.nf1 + col - g.N1 / 2;
+    int w2 = row >= g.N2 / 2 ? row - g.N2 / 2 : g.nf2 + row - g.N2 / 2;
+    double nm1, n;
+    pixel_n(row, col, g, nm1, n);
+    double zp = (nm1 - g.D) * g.gamma * g.h;
+    double s, c, ss, cs;
+    sincos(zp * (double)(p0 - g.P / 2), &s, &c);
+    sincos(zp, &ss, &cs);
+    double ar = 0.0, ai = 0.0;
+    const v2_t<T> *src = grid + (size_t)w2 * g.nf1 + w1;
+    for (int k = 0; k < nb; ++k) {
+      v2_t<T> f = src[(size_t)k * plane];
+      double fr = (double)f.x, fi = (double)f.y;
+      ar += fr * c - fi * s;
+      ai += fr * s + fi * c;
+      double cn = c * cs - s * ss;
+      s = s * cs + c * ss;
+      c = cn;
+    }
+    if (first) {
+      acc[idx] = make_double2(ar, ai);
+    } else {
+      double2 o = acc[idx];
+      acc[idx] = make_double2(o.x + ar, o.y + ai);
+    }
+  }
+}
+
+// phihat3(z') = sum_n g_n cos(x_n z')  (src/fourier/deconv.cu:45-58)
+__device__ __forceinline__ double phihat3(double zp, const double *__restrict__ q, int nq) {
+  double s = 0.0;
+  for (int i = 0; i < nq; ++i) s += q[nq + i] * cos(q[i] * zp);
+  return s;
+}
+
+// Corrections for pixel (row, col): 1 / (phihat1 phihat2 phihat3 n) and the
+// w-term phase C (z - D)  (deconv.cu:236-276, 371-399; ra_exec.cu:22-37).
+__device__ __forceinline__ void pixel_correction(int row, int col, const Geom &g, const double *__restrict__ fk1,
+                                                 const double *__restrict__ fk2, const double *__restrict__ q,
+                                                 double &scale, double &phase) {
+  double nm1, n;
+  pixel_n(row, col, g, nm1, n);
+  double zp = (nm1 - g.D) * g.gamma * g.h;
+  double ker = fk1[abs(col - g.N1 / 2)] * fk2[abs(row - g.N2 / 2)] * phihat3(zp, q, g.nquad);
+  scale = 1.0 / (ker * n);
+  phase = (nm1 - g.D) * g.C;
+}
+
+template <typename TO>
+__global__ void __launch_bounds__(256) epilogue_kernel(const double2 *__restrict__ acc, void *__restrict__ out,
+                                                        Geom g, const double *__restrict__ fk1,
+                                                        const double *__restrict__ fk2,
+                                                        const double *__restrict__ q, int accumulate) {
+  const long npix = (long)g.N1 * g.N2;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < npix; idx += (long)gridDim.x * blockDim.x) {
+    int row = (int)(idx / g.N1), col = (int)(idx - (long)row * g.N1);
+    double scale, phase, s, c;
+    pixel_correction(row, col, g, fk1, fk2, q, scale, phase);
+    sincos(phase, &s, &c);
+    double2 a = acc[idx];
+    double re = (a.x * c - a.y * s) * scale, im = (a.x * s + a.y * c) * scale;
+    if (accumulate)
+      add_c<TO>(out, idx, re, im);
+    else
+      store_c<TO>(out, idx, re, im);
+  }
+}
+
+// dirty2ms prologue: g[pix] = dirty / n e^{-i C (z - D)} / (phihat1 phihat2 phihat3)
+template <typename TI>
+__global__ void __launch_bounds__(256) prologue_kernel(const void *__restrict__ dirty, double2 *__restrict__ gimg,
+                                                        Geom g, const double *__restrict__ fk1,
+                                                        const double *__restrict__ fk2,
+                                                        const double *__restrict__ q) {
+  const long npix = (long)g.N1 * g.N2;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < npix; idx += (long)gridDim.x * blockDim.x) {
+    int row = (int)(idx / g.N1), col = (int)(idx - (long)row * g.N1);
+    double scale, phase, s, c;
+    pixel_correction(row, col, g, fk1, fk2, q, scale, phase);
+    sincos(-phase, &s, &c);
+    double2 a = load_c<TI>(dirty, idx);
+    gimg[idx] = make_double2((a.x * c - a.y * s) * scale, (a.x * s + a.y * c) * scale);
+  }
+}
+
+// w-fill (dirty2ms): plane p = zero-padded image x e^{-i z'(p - P/2)} (cuft.cu:575-612)
+template <typename T>
+__global__ void __launch_bounds__(256) wfill_kernel(const double2 *__restrict__ gimg, v2_t<T> *__restrict__ grid,
+                                                     Geom g, int p0, int nb) {
+  const long ncell = (long)g.nf1 * g.nf2;
+  const int h1 = g.N1 / 2, h2 = g.N2 / 2;
+  for (long cell = blockIdx.x * (long)blockDim.x + threadIdx.x; cell < ncell; cell += (long)gridDim.x * blockDim.x) {
+    int k2 = (int)(cell / g.nf1), k1 = (int)(cell - (long)k2 * g.nf1);
+    int col = k1 < g.N1 - h1 ? k1 + h1 : (k1 >= g.nf1 - h1 ? k1 - g.nf1 + h1 : -1);
+    int row = k2 < g.N2 - h2 ? k2 + h2 : (k2 >= g.nf2 - h2 ? k2 - g.nf2 + h2 : -1);
+    v2_t<T> *dst = grid + cell;
+    if (col < 0 || row < 0) {
+      v2_t<T> z;
+      z.x = 0;
+      z.y = 0;
+      for (int k = 0; k < nb; ++k) dst[(size_t)k * ncell] = z;
+      continue;
+    }
+    double2 v = gimg[(long)row * g.N1 + col];
+    double nm1, n;
+    pixel_n(row, col, g, nm1, n);
+    double zp = (nm1 - g.D) * g.gamma * g.h;
+    double s, c, ss, cs;
+    sincos(-zp * (double)(p0 - g.P / 2), &s, &c);
+    sincos(-zp, &ss, &cs);
+    for (int k = 0; k < nb; ++k) {
+      v2_t<T> o;
+      o.x = (T)(v.x * c - v.y * s);
+      o.y = (T)(v.x * s + v.y * c);
+      dst[(size_t)k * ncell] = o;
+      double cn = c * cs - s * ss;
+      s = s * cs + c * ss;
+      c = cn;
+    }
+  }
+}
+
+// interpolation (dirty2ms): per record, sum over the kernel support inside the
+// batch's planes; FP32 (or FP64) multiply-accumulate per plane, FP64 across planes.
+// src/fourier/interp.cu:143-228.
+template <typename T, int KWM>
+__global__ void __launch_bounds__(256) interp_kernel(const Rec<T> *__restrict__ recs,
+                                                      const v2_t<T> *__restrict__ grid,
+                                                      double2 *__restrict__ vacc, Geom g, int p0, int nb,
+                                                      int ra0, int ra1, int rb0, int rb1) {
+  const long n0 = ra1 - ra0, ntot = n0 + (rb1 - rb0);
+  const size_t plane = (size_t)g.nf1 * g.nf2;
+  const int kw = g.kw;
+  const T beta = (T)g.beta, cc = (T)g.c, hw = (T)(0.5 * kw);
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < ntot; t += (long)gridDim.x * blockDim.x) {
+    const int r = t < n0 ? ra0 + (int)t : rb0 + (int)(t - n0);
+    const Rec<T> rc = recs[r];
+    T kx[KWM], ky[KWM];
+    bool any = false;
+    for (int k = 0; k < kw; ++k) {
+      int q = rc.iw + k;
+      if (q >= g.P) q -= g.P;
+      if (q >= p0 && q < p0 + nb) any = true;
+    }
+    if (!any) continue;
+#pragma unroll
+    for (int i = 0; i < KWM; ++i) {
+      kx[i] = (i < kw) ? es_eval(rc.dx + (T)i, beta, cc, hw) : (T)0;
+      ky[i] = (i < kw) ? es_eval(rc.dy + (T)i, beta, cc, hw) : (T)0;
+    }
+    double sr = 0.0, si = 0.0;
+    for (int k = 0; k < kw; ++k) {
+      int q = rc.iw + k;
+      if (q >= g.P) q -= g.P;
+      if (q < p0 || q >= p0 + nb) continue;
+      const T kz = es_eval(rc.dw + (T)k, beta, cc, hw);
+      const v2_t<T> *pb = grid + (size_t)(q - p0) * plane;
+      T pr = 0, pi = 0;
+#pragma unroll
+      for (int jj = 0; jj < KWM; ++jj) {
+        if (jj >= kw) break;
+        int cy = rc.iy + jj;
+        if (cy >= g.nf2) cy -= g.nf2;
+        const v2_t<T> *rowp = pb + (size_t)cy * g.nf1;
+        T rr = 0, ri = 0;
+#pragma unroll
+        for (int i = 0; i < KWM; ++i) {
+          if (i >= kw) break;
+          int cx = rc.ix + i;
+          if (cx >= g.nf1) cx -= g.nf1;
+          v2_t<T> f = rowp[cx];
+          rr += f.x * kx[i];
+          ri += f.y * kx[i];
+        }
+        pr += rr * ky[jj];
+        pi += ri * ky[jj];
+      }
+      sr += (double)(pr * kz);
+      si += (double)(pi * kz);
+    }
+    double2 o = vacc[r];
+    vacc[r] = make_double2(o.x + sr, o.y + si);
+  }
+}
+
+// finalize (dirty2ms): vis_j = acc * wgt_j * e^{-i D w_j}  (ra_exec.cu:56-89)
+template <typename T>
+__global__ void __launch_bounds__(256) finalize_kernel(const int *__restrict__ perm, const double2 *__restrict__ vacc,
+                                                        int nrec, const double *__restrict__ uvw,
+                                                        const void *__restrict__ wgt, long ws,
+                                                        void *__restrict__ vis, long vs, Geom g) {
+  for (long s = blockIdx.x * (long)blockDim.x + threadIdx.x; s < nrec; s += (long)gridDim.x * blockDim.x) {
+    long j = perm[s];
+    double we = uvw[3 * j + 2] * g.wscale;
+    double sn, cs;
+    sincos(-g.D * we, &sn, &cs);
+    double wt = wgt ? load_r<T>(wgt, j * ws) : 1.0;
+    double2 a = vacc[s];
+    store_c<T>(vis, j * vs, wt * (a.x * cs - a.y * sn), wt * (a.x * sn + a.y * cs));
+  }
+}
+
+// FP64 direct sum at selected pixels (explicit_gridder, precomp.cu:242-275), with
+// the weight applied and flagged rows skipped.  One block per pixel.
+template <typename TI>
+__global__ void __launch_bounds__(256) explicit_kernel(const double *__restrict__ uvw, int nrow,
+                                                        const void *__restrict__ vis, long vs,
+                                                        const void *__restrict__ wgt, long ws,
+                                                        const uint8_t *__restrict__ flag, long fs, Geom g,
+                                                        double fc2pi, int sign, const int64_t *__restrict__ pix,
+                                                        long npix, double2 *__restrict__ out) {
+  __shared__ double sre[256], sim[256];
+  for (long q = blockIdx.x; q < npix; q += gridDim.x) {
+    long p = pix ? pix[q] : q;
+    int i = (int)(p / g.N1), k = (int)(p - (long)i * g.N1);
+    double nm1, n;
+    pixel_n(i, k, g, nm1, n);
+    double l = (double)(i - g.N2 / 2) * g.dx, m = (double)(k - g.N1 / 2) * g.dy;
+    double re = 0.0, im = 0.0;
+    for (long j = threadIdx.x; j < nrow; j += blockDim.x) {
+      if (row_skipped(flag, fs, j)) continue;
+      double ph = fc2pi * (l * uvw[3 * j] + m * uvw[3 * j + 1] + sign * uvw[3 * j + 2] * nm1);
+      double s, c;
+      sincos(ph, &s, &c);
+      double2 v = load_c<TI>(vis, j * vs);
+      double wt = wgt ? load_r<TI>(wgt, j * ws) : 1.0;
+      re += wt * (v.x * c - v.y * s);
+      im += wt * (v.x * s + v.y * c);
+    }
+    sre[threadIdx.x] = re;
+    sim[threadIdx.x] = im;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+      if (threadIdx.x < st) {
+        sre[threadIdx.x] += sre[threadIdx.x + st];
+        sim[threadIdx.x] += sim[threadIdx.x + st];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = make_double2(sre[0] / n, sim[0] / n);
+    __syncthreads();
+  }
+}
+
+// ==========================================================================
+// host side
+// ==========================================================================
+namespace {
+
+// Grow-only device buffers, one set per device (calls on one device are serialised).
+enum Slot {
+  S_PART, S_CNT, S_HIST, S_OFF, S_BSUM, S_BINRANK, S_RECS, S_PERM, S_GRID, S_ACC, S_VACC, S_TAB, S_FFTW,
+  S_NSLOT
+};
+struct Workspace {
+  void *p[S_NSLOT] = {};
+  size_t n[S_NSLOT] = {};
+};
+std::mutex g_ws_mu;
+std::vector<Workspace> g_ws;
+
+int ws_get(int dev, Slot s, size_t bytes, void **out) {
+  if ((int)g_ws.size() <= dev) g_ws.resize(dev + 1);
+  Workspace &w = g_ws[dev];
+  if (bytes == 0) bytes = 16;
+  if (w.n[s] < bytes) {
+    if (w.p[s]) {
+      CURIG_HIP(hipFree(w.p[s]));
+      w.p[s] = nullptr;
+      w.n[s] = 0;
+    }
+    hipError_t e = hipMalloc(&w.p[s], bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+      return kErrAlloc;
+    }
+    w.n[s] = bytes;
+  }
+  *out = w.p[s];
+  return kOk;
+}
+
+inline int imod_host(int a, int n) {
+  int r = a % n;
+  return r < 0 ? r + n : r;
+}
+
+inline int nblocks(long n, int bs = 256, long cap = 1 << 20) {
+  long b = (n + bs - 1) / bs;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (int)b;
+}
+
+struct Timer {
+  bool on = false;
+  hipStream_t s = nullptr;
+  std::vector<hipEvent_t> ev;
+  int mark() {
+    if (!on) return -1;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return -1;
+    hipEventRecord(e, s);
+    ev.push_back(e);
+    return (int)ev.size() - 1;
+  }
+  double ms(int a, int b) {
+    if (!on || a < 0 || b < 0) return 0;
+    float t = 0;
+    hipEventElapsedTime(&t, ev[a], ev[b]);
+    return t;
+  }
+  ~Timer() {
+    for (auto e : ev) hipEventDestroy(e);
+  }
+};
+
+int check_args(const Problem &pb) {
+  if (pb.nrow < 0) {
+    set_error("nrow %d < 0", pb.nrow);
+    return kErrNrow;
+  }
+  if (pb.nx <= 0 || pb.ny <= 0) {
+    set_error("image size %d x %d invalid", pb.nx, pb.ny);
+    return kErrSize;
+  }
+  if (pb.sign != 1 && pb.sign != -1) {
+    set_error("sign must be +1 or -1 (got %d)", pb.sign);
+    return kErrArg;
+  }
+  if (!(std::isfinite(pb.fov) && pb.fov > 0) || !(std::isfinite(pb.freq) && pb.freq > 0)) {
+    set_error("fov (%g) and freq (%g) must be positive and finite", pb.fov, pb.freq);
+    return kErrNonFinite;
+  }
+  if (!pb.dirty || (pb.nrow > 0 && (!pb.uvw || !pb.vis))) {
+    set_error("NULL data pointer");
+    return kErrArg;
+  }
+  double dx = pb.fov * M_PI / 180.0 / pb.nx, dy = pb.fov * M_PI / 180.0 / pb.ny;
+  double lmax = 0.5 * dx * pb.nx, mmax = 0.5 * dy * pb.ny;
+  if (!(lmax * lmax + mmax * mmax < 1.0)) {
+    set_error("field of view %g deg reaches beyond the horizon (l^2 + m^2 >= 1)", pb.fov);
+    return kErrArg;
+  }
+  return kOk;
+}
+
+}  // namespace
+
+int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, double eps, double sigma, int sign,
+                  double wmin_m, double wmax_m, bool dbl, const Tuning *tune, Geom *gp, KernelOpts *kop) {
+  KernelOpts ko;
+  int ier = setup_conv_opts(ko, eps, sigma);
+  if (ier != kOk) return ier;
+  Geom g{};
+  g.N1 = ny;
+  g.N2 = nx;
+  g.kw = ko.kw;
+  g.beta = ko.beta;
+  g.c = ko.c;
+  // src/astro/cugridder.cu:48-49 (N2 = nx pairs with u, N1 = ny with v)
+  g.dx = fov / 180.0 * M_PI / (double)nx;
+  g.dy = fov / 180.0 * M_PI / (double)ny;
+  double fc = freq / kSpeedOfLight;
+  g.uscale = fc * 2 * M_PI * g.dx;
+  g.vscale = fc * 2 * M_PI * g.dy;
+  g.wscale = fc * sign * 2 * M_PI;
+  // type-3 parameters (cugridder.cu:52-76)
+  double l_min = -0.5 * g.dx * nx, m_min = -0.5 * g.dy * ny;
+  double n_lm = std::sqrt(1.0 - l_min * l_min - m_min * m_min);
+  double we0 = wmin_m * g.wscale, we1 = wmax_m * g.wscale;
+  double i_max = nrow_valid > 0 ? std::max(we0, we1) : 0.0;
+  double i_min = nrow_valid > 0 ? std::min(we0, we1) : 0.0;
+  g.C = (i_max + i_min) / 2.0;
+  double X = (i_max - i_min) / 2.0;
+  double o_min = n_lm - 1;
+  g.D = o_min / 2.0;
+  double S = std::fabs(o_min / 2.0);
+  set_nhg_type3(S, X, ko, g.P, g.h, g.gamma);
+  g.nf1 = get_num_cells(ny, ko);
+  g.nf2 = get_num_cells(nx, ko);
+  if ((double)g.nf1 * g.nf2 > 2.0e9) {
+    set_error("grid %d x %d too large", g.nf1, g.nf2);
+    return kErrSize;
+  }
+  Tuning tn = tune ? *tune : Tuning();
+  int T1 = tn.T1 > 0 ? tn.T1 : (dbl ? 32 : 64);
+  int T2 = tn.T2 > 0 ? tn.T2 : 32;
+  g.T1 = std::min(T1, g.nf1);
+  g.T2 = std::min(T2, g.nf2);
+  g.ntx = (g.nf1 + g.T1 - 1) / g.T1;
+  g.nty = (g.nf2 + g.T2 - 1) / g.T2;
+  g.ntiles = g.ntx * g.nty;
+  size_t plane = (size_t)g.nf1 * g.nf2 * (dbl ? 16 : 8);
+  int Pb = tn.Pb;
+  if (Pb <= 0) {
+    Pb = (int)std::max<size_t>(1, tn.grid_budget_bytes / plane);
+    Pb = std::min(Pb, 32);
+  }
+  g.Pb = std::max(1, std::min(Pb, g.P));
+  g.nbatch = (g.P + g.Pb - 1) / g.Pb;
+  if ((double)g.nbatch * g.ntiles > 1.5e9) {
+    set_error("too many bins (%d batches x %d tiles)", g.nbatch, g.ntiles);
+    return kErrSize;
+  }
+  double qx[kMaxQuadNodes], qg[kMaxQuadNodes];
+  g.nquad = kernel_quadrature(ko, qx, qg);
+  *gp = g;
+  if (kop) *kop = ko;
+  return kOk;
+}
+
+namespace {
+
+// Shared prep: w range -> geometry; bins; sorted records.  Returns number of records.
+template <typename T>
+int prepare(const Problem &pb, hipStream_t st, const Tuning *tune, int dev, Geom &g, KernelOpts &ko,
+            Rec<T> *&recs, int *&perm, int *&off, int &nrec, double *&tabs, bool with_value) {
+  const int nrow = pb.nrow;
+  const int nbm = std::min(nblocks(nrow), 1024);
+  void *vp;
+  CURIG_TRY(ws_get(dev, S_PART, sizeof(double) * 2 * nbm, &vp));
+  double *part = (double *)vp;
+  CURIG_TRY(ws_get(dev, S_CNT, sizeof(int) * nbm, &vp));
+  int *cnt = (int *)vp;
+  double wmin = 0, wmax = 0;
+  long nvalid = 0;
+  if (nrow > 0) {
+    w_minmax_kernel<<<nbm, 256, 0, st>>>(pb.uvw, nrow, pb.flag, pb.flag_stride, part, cnt);
+    CURIG_HIP(hipGetLastError());
+    std::vector<double> hp(2 * nbm);
+    std::vector<int> hc(nbm);
+    CURIG_HIP(hipMemcpyAsync(hp.data(), part, sizeof(double) * 2 * nbm, hipMemcpyDeviceToHost, st));
+    CURIG_HIP(hipMemcpyAsync(hc.data(), cnt, sizeof(int) * nbm, hipMemcpyDeviceToHost, st));
+    CURIG_HIP(hipStreamSynchronize(st));
+    wmin = INFINITY;
+    wmax = -INFINITY;
+    bool bad = false;
+    for (int b = 0; b < nbm; ++b) {
+      nvalid += hc[b];
+      if (hc[b] == 0) continue;
+      if (std::isnan(hp[2 * b]) || std::isnan(hp[2 * b + 1])) bad = true;
+      wmin = std::min(wmin, hp[2 * b]);
+      wmax = std::max(wmax, hp[2 * b + 1]);
+    }
+    if (bad) {
+      set_error("non-finite uvw coordinates");
+      return kErrNonFinite;
+    }
+    if (nvalid == 0) wmin = wmax = 0;
+  }
+  CURIG_TRY(plan_geometry((int)nvalid, pb.nx, pb.ny, pb.fov, pb.freq, pb.eps, pb.sigma, pb.sign, wmin, wmax, pb.dbl,
+                          tune, &g, &ko));
+  // correction tables: fk1 [nf1/2+1], fk2 [nf2/2+1], quadrature nodes and weights [2 nquad]
+  size_t n1 = g.nf1 / 2 + 1, n2 = g.nf2 / 2 + 1;
+  std::vector<double> ht(n1 + n2 + 2 * kMaxQuadNodes);
+  fseries_grid(ko, g.nf1, ht.data());
+  fseries_grid(ko, g.nf2, ht.data() + n1);
+  kernel_quadrature(ko, ht.data() + n1 + n2, ht.data() + n1 + n2 + g.nquad);
+  CURIG_TRY(ws_get(dev, S_TAB, sizeof(double) * ht.size(), &vp));
+  tabs = (double *)vp;
+  CURIG_HIP(hipMemcpyAsync(tabs, ht.data(), sizeof(double) * ht.size(), hipMemcpyHostToDevice, st));
+
+  const long nbins = (long)g.nbatch * g.ntiles;
+  CURIG_TRY(ws_get(dev, S_HIST, sizeof(int) * (nbins + 1), &vp));
+  int *hist = (int *)vp;
+  CURIG_TRY(ws_get(dev, S_OFF, sizeof(int) * (nbins + 1), &vp));
+  off = (int *)vp;
+  const long nscan = (nbins + kScanTile - 1) / kScanTile;
+  CURIG_TRY(ws_get(dev, S_BSUM, sizeof(int) * (nscan + 1), &vp));
+  int *bsum = (int *)vp;
+  CURIG_TRY(ws_get(dev, S_BINRANK, sizeof(int2) * std::max(nrow, 1), &vp));
+  int2 *binrank = (int2 *)vp;
+  CURIG_TRY(ws_get(dev, S_RECS, sizeof(Rec<T>) * std::max(nrow, 1), &vp));
+  recs = (Rec<T> *)vp;
+  CURIG_TRY(ws_get(dev, S_PERM, sizeof(int) * std::max(nrow, 1), &vp));
+  perm = (int *)vp;
+
+  fill_int_kernel<<<nblocks(nbins), 256, 0, st>>>(hist, nbins, 0);
+  if (nrow > 0)
+    bin_count_kernel<<<nblocks(nrow), 256, 0, st>>>(pb.uvw, nrow, pb.flag, pb.flag_stride, g, hist, binrank);
+  scan_reduce_kernel<<<(int)nscan, kScanBlock, 0, st>>>(hist, nbins, bsum);
+  scan_sums_kernel<<<1, 1024, 0, st>>>(bsum, (int)nscan);
+  scan_apply_kernel<<<(int)nscan, kScanBlock, 0, st>>>(hist, nbins, bsum, off, (int)nvalid);
+  if (nrow > 0)
+    bin_scatter_kernel<T><<<nblocks(nrow), 256, 0, st>>>(pb.uvw, nrow, pb.vis, pb.vis_stride, pb.wgt,
+                                                          pb.wgt_stride, g, binrank, off, recs, perm,
+                                                          with_value ? 1 : 0);
+  CURIG_HIP(hipGetLastError());
+  nrec = (int)nvalid;
+  return kOk;
+}
+
+template <typename T, int KWM>
+void launch_spread(const Rec<T> *recs, const int *off, v2_t<T> *grid, const Geom &g, int p0, int nb,
+                   hipStream_t st) {
+  dim3 grd(g.ntiles, nb);
+  size_t lds = sizeof(T) * 2 * g.T1 * g.T2;
+  spread_tile_kernel<T, KWM><<<grd, 256, lds, st>>>(recs, off, grid, g, p0);
+}
+
+template <typename T>
+void spread_dispatch(const Rec<T> *recs, const int *off, v2_t<T> *grid, const Geom &g, int p0, int nb,
+                     hipStream_t st) {
+  if (g.kw <= 8)
+    launch_spread<T, 8>(recs, off, grid, g, p0, nb, st);
+  else if (g.kw <= 16)
+    launch_spread<T, 16>(recs, off, grid, g, p0, nb, st);
+  else
+    launch_spread<T, kMaxKernelWidth>(recs, off, grid, g, p0, nb, st);
+}
+
+template <typename T>
+void interp_dispatch(const Rec<T> *recs, const v2_t<T> *grid, double2 *vacc, const Geom &g, int p0, int nb,
+                     int ra0, int ra1, int rb0, int rb1, hipStream_t st) {
+  long n = (long)(ra1 - ra0) + (rb1 - rb0);
+  if (n <= 0) return;
+  int nbk = nblocks(n);
+  if (g.kw <= 8)
+    interp_kernel<T, 8><<<nbk, 256, 0, st>>>(recs, grid, vacc, g, p0, nb, ra0, ra1, rb0, rb1);
+  else if (g.kw <= 16)
+    interp_kernel<T, 16><<<nbk, 256, 0, st>>>(recs, grid, vacc, g, p0, nb, ra0, ra1, rb0, rb1);
+  else
+    interp_kernel<T, kMaxKernelWidth><<<nbk, 256, 0, st>>>(recs, grid, vacc, g, p0, nb, ra0, ra1, rb0, rb1);
+}
+
+// FFT for a batch of nb planes (full batches share one plan, the tail has its own,
+// like fftplan / fftplan_l in the reference, cugridder.cu:230-246).
+template <typename T>
+int batch_fft(int dev, const Geom &g, int nb, int sign, void *grid, hipStream_t st) {
+  FftPlan fp;
+  CURIG_TRY(fft_plan(dev, g.nf1, g.nf2, nb, sizeof(T) == 8, sign, &fp));
+  void *work = nullptr;
+  if (fp.work_bytes) CURIG_TRY(ws_get(dev, S_FFTW, fp.work_bytes, &work));
+  return fft_exec(fp, grid, work, st);
+}
+
+// rows whose first plane lies in [lo, hi] (cyclic, length <= P) -> up to two record ranges
+void source_ranges(const Geom &g, const std::vector<int> &hoff, int lo, int len, int &a0, int &a1, int &b0,
+                   int &b1) {
+  // batches touched by the cyclic plane interval [lo, lo+len)
+  if (len > g.P) len = g.P;
+  int first = lo / g.Pb;
+  int last_plane = lo + len - 1;
+  a0 = a1 = b0 = b1 = 0;
+  if (last_plane < g.P) {
+    int lastb = last_plane / g.Pb;
+    a0 = hoff[first];
+    a1 = hoff[lastb + 1];
+  } else {
+    a0 = hoff[first];
+    a1 = hoff[g.nbatch];
+    int lastb = (last_plane - g.P) / g.Pb;
+    if (lastb >= first) lastb = first - 1;  // whole circle already covered
+    if (lastb >= 0) {
+      b0 = hoff[0];
+      b1 = hoff[lastb + 1];
+    }
+  }
+}
+
+template <typename T>
+int ms2dirty_impl(const Problem &pb, hipStream_t st, const Tuning *tune, Stats *sts) {
+  CURIG_TRY(check_args(pb));
+  int dev = 0;
+  CURIG_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  Timer tm;
+  tm.on = sts != nullptr;
+  tm.s = st;
+  int e0 = tm.mark();
+  Geom g;
+  KernelOpts ko;
+  Rec<T> *recs;
+  int *perm, *off, nrec;
+  double *tabs;
+  CURIG_TRY(prepare<T>(pb, st, tune, dev, g, ko, recs, perm, off, nrec, tabs, true));
+  int e1 = tm.mark();
+  void *vp;
+  const size_t plane = (size_t)g.nf1 * g.nf2;
+  CURIG_TRY(ws_get(dev, S_GRID, sizeof(v2_t<T>) * plane * g.Pb, &vp));
+  v2_t<T> *grid = (v2_t<T> *)vp;
+  const long npix = (long)g.N1 * g.N2;
+  CURIG_TRY(ws_get(dev, S_ACC, sizeof(double2) * npix, &vp));
+  double2 *acc = (double2 *)vp;
+  const double *fk1 = tabs, *fk2 = tabs + (g.nf1 / 2 + 1), *q = fk2 + (g.nf2 / 2 + 1);
+  double t_spread = 0, t_fft = 0, t_wacc = 0;
+  for (int b = 0; b < g.nbatch; ++b) {
+    int p0 = b * g.Pb, nb = std::min(g.Pb, g.P - p0);
+    int a = tm.mark();
+    spread_dispatch<T>(recs, off, grid, g, p0, nb, st);
+    CURIG_HIP(hipGetLastError());
+    int bm = tm.mark();
+    CURIG_TRY(batch_fft<T>(dev, g, nb, +1, grid, st));
+    int c = tm.mark();
+    wacc_kernel<T><<<nblocks(npix), 256, 0, st>>>(grid, acc, g, p0, nb, b == 0 ? 1 : 0);
+    CURIG_HIP(hipGetLastError());
+    int d = tm.mark();
+    if (tm.on) {
+      CURIG_HIP(hipEventSynchronize(tm.ev[d]));
+      t_spread += tm.ms(a, bm);
+      t_fft += tm.ms(bm, c);
+      t_wacc += tm.ms(c, d);
+    }
+  }
+  int e2 = tm.mark();
+  if (pb.dbl)
+    epilogue_kernel<double><<<nblocks(npix), 256, 0, st>>>(acc, pb.dirty, g, fk1, fk2, q, pb.accumulate ? 1 : 0);
+  else
+    epilogue_kernel<float><<<nblocks(npix), 256, 0, st>>>(acc, pb.dirty, g, fk1, fk2, q, pb.accumulate ? 1 : 0);
+  CURIG_HIP(hipGetLastError());
+  int e3 = tm.mark();
+  if (sts) {
+    CURIG_HIP(hipStreamSynchronize(st));
+    sts->kw = g.kw; sts->nf1 = g.nf1; sts->nf2 = g.nf2; sts->P = g.P; sts->Pb = g.Pb; sts->nbatch = g.nbatch;
+    sts->T1 = g.T1; sts->T2 = g.T2; sts->nvis = nrec; sts->beta = g.beta; sts->gamma = g.gamma;
+    sts->C = g.C; sts->D = g.D;
+    sts->t_prep = tm.ms(e0, e1);
+    sts->t_grid = t_spread;
+    sts->t_fft = t_fft;
+    sts->t_plane = t_wacc;
+    sts->t_final = tm.ms(e2, e3);
+    sts->t_total = tm.ms(e0, e3);
+    sts->timed = 1;
+  }
+  return kOk;
+}
+
+template <typename T>
+int dirty2ms_impl(const Problem &pb, hipStream_t st, const Tuning *tune, Stats *sts) {
+  CURIG_TRY(check_args(pb));
+  int dev = 0;
+  CURIG_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  Timer tm;
+  tm.on = sts != nullptr;
+  tm.s = st;
+  int e0 = tm.mark();
+  Geom g;
+  KernelOpts ko;
+  Rec<T> *recs;
+  int *perm, *off, nrec;
+  double *tabs;
+  CURIG_TRY(prepare<T>(pb, st, tune, dev, g, ko, recs, perm, off, nrec, tabs, false));
+  // every output row is written: flagged rows get 0
+  if (pb.nrow > 0) {
+    if (pb.dbl)
+      zero_rows_kernel<double><<<nblocks(pb.nrow), 256, 0, st>>>(pb.vis, pb.vis_stride, pb.nrow);
+    else
+      zero_rows_kernel<float><<<nblocks(pb.nrow), 256, 0, st>>>(pb.vis, pb.vis_stride, pb.nrow);
+  }
+  std::vector<int> hoff(g.nbatch + 1);
+  for (int b = 0; b <= g.nbatch; ++b)
+    CURIG_HIP(hipMemcpyAsync(&hoff[b], off + (long)b * g.ntiles, sizeof(int), hipMemcpyDeviceToHost, st));
+  void *vp;
+  const size_t plane = (size_t)g.nf1 * g.nf2;
+  CURIG_TRY(ws_get(dev, S_GRID, sizeof(v2_t<T>) * plane * g.Pb, &vp));
+  v2_t<T> *grid = (v2_t<T> *)vp;
+  const long npix = (long)g.N1 * g.N2;
+  CURIG_TRY(ws_get(dev, S_ACC, sizeof(double2) * npix, &vp));
+  double2 *gimg = (double2 *)vp;
+  CURIG_TRY(ws_get(dev, S_VACC, sizeof(double2) * std::max(nrec, 1), &vp));
+  double2 *vacc = (double2 *)vp;
+  const double *fk1 = tabs, *fk2 = tabs + (g.nf1 / 2 + 1), *q = fk2 + (g.nf2 / 2 + 1);
+  if (pb.dbl)
+    prologue_kernel<double><<<nblocks(npix), 256, 0, st>>>(pb.dirty, gimg, g, fk1, fk2, q);
+  else
+    prologue_kernel<float><<<nblocks(npix), 256, 0, st>>>(pb.dirty, gimg, g, fk1, fk2, q);
+  fill_zero_c128_kernel<<<nblocks(nrec), 256, 0, st>>>(vacc, nrec);
+  CURIG_HIP(hipGetLastError());
+  CURIG_HIP(hipStreamSynchronize(st));  // hoff
+  int e1 = tm.mark();
+  double t_interp = 0, t_fft = 0, t_fill = 0;
+  for (int b = 0; b < g.nbatch; ++b) {
+    int p0 = b * g.Pb, nb = std::min(g.Pb, g.P - p0);
+    int a = tm.mark();
+    wfill_kernel<T><<<nblocks((long)plane), 256, 0, st>>>(gimg, grid, g, p0, nb);
+    CURIG_HIP(hipGetLastError());
+    int bm = tm.mark();
+    CURIG_TRY(batch_fft<T>(dev, g, nb, -1, grid, st));
+    int c = tm.mark();
+    int ra0, ra1, rb0, rb1;
+    source_ranges(g, hoff, imod_host(p0 - g.kw + 1, g.P), nb + g.kw - 1, ra0, ra1, rb0, rb1);
+    interp_dispatch<T>(recs, grid, vacc, g, p0, nb, ra0, ra1, rb0, rb1, st);
+    CURIG_HIP(hipGetLastError());
+    int d = tm.mark();
+    if (tm.on) {
+      CURIG_HIP(hipEventSynchronize(tm.ev[d]));
+      t_fill += tm.ms(a, bm);
+      t_fft += tm.ms(bm, c);
+      t_interp += tm.ms(c, d);
+    }
+  }
+  int e2 = tm.mark();
+  if (nrec > 0) {
+    if (pb.dbl)
+      finalize_kernel<double><<<nblocks(nrec), 256, 0, st>>>(perm, vacc, nrec, pb.uvw, pb.wgt, pb.wgt_stride,
+                                                              pb.vis, pb.vis_stride, g);
+    else
+      finalize_kernel<float><<<nblocks(nrec), 256, 0, st>>>(perm, vacc, nrec, pb.uvw, pb.wgt, pb.wgt_stride,
+                                                             pb.vis, pb.vis_stride, g);
+    CURIG_HIP(hipGetLastError());
+  }
+  int e3 = tm.mark();
+  if (sts) {
+    CURIG_HIP(hipStreamSynchronize(st));
+    sts->kw = g.kw; sts->nf1 = g.nf1; sts->nf2 = g.nf2; sts->P = g.P; sts->Pb = g.Pb; sts->nbatch = g.nbatch;
+    sts->T1 = g.T1; sts->T2 = g.T2; sts->nvis = nrec; sts->beta = g.beta; sts->gamma = g.gamma;
+    sts->C = g.C; sts->D = g.D;
+    sts->t_prep = tm.ms(e0, e1);
+    sts->t_grid = t_interp;
+    sts->t_fft = t_fft;
+    sts->t_plane = t_fill;
+    sts->t_final = tm.ms(e2, e3);
+    sts->t_total = tm.ms(e0, e3);
+    sts->timed = 1;
+  }
+  return kOk;
+}
+
+}  // namespace
+
+int ms2dirty_device(const Problem &pb, hipStream_t st, const Tuning *tune, Stats *sts) {
+  return pb.dbl ? ms2dirty_impl<double>(pb, st, tune, sts) : ms2dirty_impl<float>(pb, st, tune, sts);
+}
+
+int dirty2ms_device(const Problem &pb, hipStream_t st, const Tuning *tune, Stats *sts) {
+  return pb.dbl ? dirty2ms_impl<double>(pb, st, tune, sts) : dirty2ms_impl<float>(pb, st, tune, sts);
+}
+
+int explicit_ms2dirty_device(const Problem &pb, long npix, const int64_t *pix, void *out, hipStream_t st) {
+  CURIG_TRY(check_args(pb));
+  Geom g{};
+  g.N1 = pb.ny;
+  g.N2 = pb.nx;
+  g.dx = pb.fov / 180.0 * M_PI / (double)pb.nx;
+  g.dy = pb.fov / 180.0 * M_PI / (double)pb.ny;
+  if (pix == nullptr) npix = (long)pb.nx * pb.ny;
+  if (npix <= 0) return kOk;
+  double fc2pi = 2 * M_PI * pb.freq / kSpeedOfLight;
+  int nbk = (int)std::min<long>(npix, 1 << 16);
+  if (pb.dbl)
+    explicit_kernel<double><<<nbk, 256, 0, st>>>(pb.uvw, pb.nrow, pb.vis, pb.vis_stride, pb.wgt, pb.wgt_stride,
+                                                 pb.flag, pb.flag_stride, g, fc2pi, pb.sign, pix, npix,
+                                                 (double2 *)out);
+  else
+    explicit_kernel<float><<<nbk, 256, 0, st>>>(pb.uvw, pb.nrow, pb.vis, pb.vis_stride, pb.wgt, pb.wgt_stride,
+                                                pb.flag, pb.flag_stride, g, fc2pi, pb.sign, pix, npix,
+                                                (double2 *)out);
+  CURIG_HIP(hipGetLastError());
+  return kOk;
+}
+
+void release_workspace() {
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  for (size_t d = 0; d < g_ws.size(); ++d) {
+    (void)hipSetDevice((int)d);
+    for (int s = 0; s < S_NSLOT; ++s)
+      if (g_ws[d].p[s]) (void)hipFree(g_ws[d].p[s]);
+  }
+  g_ws.clear();
+  (void)hipSetDevice(cur);
+  fft_release_all();
+}
+
+}  // namespace curig

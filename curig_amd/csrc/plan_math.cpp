@@ -1,0 +1,165 @@
+// Host plan math: kernel width / beta, grid sizes, w-plane count, quadrature and
+// the kernel's Fourier series.  Pure host code (no HIP calls).
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "curig_internal.h"
+
+namespace curig {
+
+namespace {
+thread_local char g_err[1024] = "";
+}
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+const char *last_error() { return g_err; }
+
+// src/fourier/conv_interp_invoker.cu:46-116.  Returns 0, 1 (eps clamped: the
+// C entry points then stop, as cugridder.cu:403-406 does) or 2 (sigma <= 1).
+int setup_conv_opts(KernelOpts &o, double eps, double upsampfac) {
+  if (!(upsampfac > 1.0)) {
+    set_error("upsampling factor %.3g too small (must be > 1)", upsampfac);
+    return kErrSigma;
+  }
+  int ier = kOk;
+  if (!(eps >= kEpsilonMin)) {
+    set_error("epsilon %.3g below %.3g", eps, kEpsilonMin);
+    eps = kEpsilonMin;
+    ier = kErrEpsTooSmall;
+  }
+  int kw = (int)std::ceil(-std::log10(eps / 10.0));
+  if (upsampfac != 2.0)
+    kw = (int)std::ceil(-std::log(eps) / (M_PI * std::sqrt(1.0 - 1.0 / upsampfac)));
+  if (kw < 2) kw = 2;
+  if (kw > kMaxKernelWidth) kw = kMaxKernelWidth;
+  double betaoverns = 2.30;
+  if (kw == 2) betaoverns = 2.20;
+  if (kw == 3) betaoverns = 2.26;
+  if (kw == 4) betaoverns = 2.38;
+  if (upsampfac != 2.0) betaoverns = 0.97 * M_PI * (1 - 1 / (2 * upsampfac));
+  o.kw = kw;
+  o.halfwidth = kw / 2.0;
+  o.c = 4.0 / (double)(kw * kw);
+  o.beta = betaoverns * kw;
+  o.upsampfac = upsampfac;
+  return ier;
+}
+
+// src/utils.cu:251-276
+int next235beven(int n, int b) {
+  if (n <= 2) return 2;
+  if (n % 2 == 1) n += 1;
+  int nplus = n - 2, numdiv = 2;
+  while (numdiv > 1 || nplus % b != 0) {
+    nplus += 2;
+    numdiv = nplus;
+    while (numdiv % 2 == 0) numdiv /= 2;
+    while (numdiv % 3 == 0) numdiv /= 3;
+    while (numdiv % 5 == 0) numdiv /= 5;
+  }
+  return nplus;
+}
+
+// src/fourier/conv_interp_invoker.cu:29-44
+int get_num_cells(int ms, const KernelOpts &o) {
+  int nf = (int)(o.upsampfac * ms);
+  if (nf < 2 * o.kw) nf = 2 * o.kw;
+  return next235beven(nf, 1);
+}
+
+// contrib/common.cpp:161-196 (S: half-width of n-1, X: half-width of w, radians)
+void set_nhg_type3(double S, double X, const KernelOpts &o, int &nf, double &h, double &gam) {
+  int nss = o.kw + 1;
+  double Xsafe = X, Ssafe = S;
+  if (X == 0.0) {
+    if (S == 0.0) {
+      Xsafe = 1.0;
+      Ssafe = 1.0;
+    } else {
+      Xsafe = std::max(Xsafe, 1 / S);
+    }
+  } else {
+    Ssafe = std::max(Ssafe, 1 / X);
+  }
+  double nfd = 2.0 * o.upsampfac * Ssafe * Xsafe / M_PI + nss;
+  if (!std::isfinite(nfd)) nfd = 0.0;
+  nf = (int)nfd;
+  if (nf < 2 * o.kw) nf = 2 * o.kw;
+  nf = next235beven(nf, 1);
+  h = 2 * M_PI / nf;
+  gam = (double)nf / (2.0 * o.upsampfac * Ssafe);
+}
+
+// n-point Gauss-Legendre rule on [-1,1], nodes ascending.  The reference uses
+// the Glaser-Liu-Rokhlin solver (contrib/legendre_rule_fast.c:15-92) for the
+// same unique rule; here: Newton on the three-term recurrence.
+void gauss_legendre(int n, double *x, double *w) {
+  for (int i = 0; i < n; ++i) {
+    double z = std::cos(M_PI * (i + 0.75) / (n + 0.5)), dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = z;
+      for (int k = 2; k <= n; ++k) {
+        double p2 = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = n * (z * p1 - p0) / (z * z - 1.0);
+      double dz = p1 / dp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-16) break;
+    }
+    double p0 = 1.0, p1 = z;
+    for (int k = 2; k <= n; ++k) {
+      double p2 = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k;
+      p0 = p1;
+      p1 = p2;
+    }
+    dp = n * (z * p1 - p0) / (z * z - 1.0);
+    x[n - 1 - i] = z;
+    w[n - 1 - i] = 2.0 / ((1.0 - z * z) * dp * dp);
+  }
+}
+
+// Scaled nodes and weighted kernel values of the 2p-node rule
+// (src/fourier/deconv.cu:165-181: alpha = kw/2, p = (int)(2 + 3 alpha)).
+// phi is the normalised kernel exp(beta (sqrt(1 - c x^2) - 1)).
+int kernel_quadrature(const KernelOpts &o, double *x, double *g) {
+  double alpha = o.kw / 2.0;
+  int p = (int)(2 + 3.0 * alpha);
+  int n = 2 * p;
+  double t[kMaxQuadNodes], wq[kMaxQuadNodes];
+  gauss_legendre(n, t, wq);
+  for (int i = 0; i < n; ++i) {
+    double xi = t[i] * alpha, phi = 0.0;
+    if (std::fabs(xi) <= o.halfwidth) {
+      double s = 1.0 - o.c * xi * xi;
+      phi = std::exp(o.beta * (std::sqrt(s > 0 ? s : 0.0) - 1.0));
+    }
+    x[i] = xi;
+    g[i] = alpha * wq[i] * phi;
+  }
+  return n;
+}
+
+// src/fourier/deconv.cu:13-24, 60-102: phihat(k) (-1)^k, k = 0..nf/2
+void fseries_grid(const KernelOpts &o, int nf, double *out) {
+  double x[kMaxQuadNodes], g[kMaxQuadNodes];
+  int n = kernel_quadrature(o, x, g);
+  int half = nf / 2;
+  for (int k = 0; k <= half; ++k) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += g[i] * std::cos((x[i] - half) / half * M_PI * k);
+    out[k] = s;
+  }
+}
+
+}  // namespace curig

@@ -1,0 +1,93 @@
+/* Additive exports of libcurafft.so (no counterpart among the reference's four
+ * C entry points; they do not change those).  Conventions and return codes as
+ * in cugridder.h.
+ *
+ *  - FP32-mixed form: complex64 visibilities / image and float32 weights; the ES
+ *    kernel and the gridding / interpolation multiply-accumulate run in FP32,
+ *    while uvw scaling, the w phase and the sum over w planes stay FP64.
+ *  - Device-resident forms: all array pointers are device pointers on the current
+ *    HIP device; work is queued on `stream` (a hipStream_t, NULL = default stream).
+ *    The call returns after queuing, except for one internal wait for the w range.
+ *  - Multi-channel forms (NIFTY layout): vis / wgt / flag are [nrow][nchan].
+ *  - precision: 0 = FP64 (complex128 / float64), 1 = FP32-mixed (complex64 / float32).
+ */
+#ifndef CURIG_EXT_H
+#define CURIG_EXT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* FP32-mixed host entry points; same arguments as ms2dirty_2 / dirty2ms_2 with
+ * vis, dirty complex64 and wgt float32 (wgt may be NULL). */
+int ms2dirty_f32(int nrow, int nxdirty, int nydirty, double fov, double freq, const double *uvw, const void *vis,
+                 const float *wgt, void *dirty, double epsilon, double sigma, int sign);
+int dirty2ms_f32(int nrow, int nxdirty, int nydirty, double fov, double freq, const double *uvw, void *vis,
+                 const float *wgt, const void *dirty, double epsilon, double sigma, int sign);
+
+/* Device-resident single channel. */
+int curig_ms2dirty_dev(int nrow, int nxdirty, int nydirty, double fov, double freq, const double *d_uvw,
+                       const void *d_vis, const void *d_wgt, void *d_dirty, double epsilon, double sigma, int sign,
+                       int precision, void *stream);
+int curig_dirty2ms_dev(int nrow, int nxdirty, int nydirty, double fov, double freq, const double *d_uvw,
+                       void *d_vis, const void *d_wgt, const void *d_dirty, double epsilon, double sigma, int sign,
+                       int precision, void *stream);
+
+/* Device-resident multi-channel.  freq: host array [nchan].  d_flag: uint8
+ * [nrow][nchan], nonzero excludes the sample (NULL: none).  ms2dirty writes
+ * nchan images [nchan][nx][ny] when per_channel != 0, else one image summed over
+ * channels.  dirty2ms reads the matching layout and writes every sample (flagged
+ * samples get 0). */
+int curig_ms2dirty_multi_dev(int nrow, int nchan, int nxdirty, int nydirty, double fov, const double *freq,
+                             const double *d_uvw, const void *d_vis, const void *d_wgt, const uint8_t *d_flag,
+                             void *d_dirty, int per_channel, double epsilon, double sigma, int sign, int precision,
+                             void *stream);
+int curig_dirty2ms_multi_dev(int nrow, int nchan, int nxdirty, int nydirty, double fov, const double *freq,
+                             const double *d_uvw, void *d_vis, const void *d_wgt, const uint8_t *d_flag,
+                             const void *d_dirty, int per_channel, double epsilon, double sigma, int sign,
+                             int precision, void *stream);
+
+/* FP64 direct sum on the GPU (the reference's explicit_gridder,
+ * src/astro/precomp.cu:242-315, with the weight applied): out[q] (complex128) =
+ * dirty pixel d_pix[q] = i*ny + k; with d_pix == NULL, every pixel (npix ignored).
+ * vis / wgt precision as `precision`. */
+int curig_explicit_ms2dirty_dev(int nrow, int nxdirty, int nydirty, double fov, double freq,
+                                const double *d_uvw, const void *d_vis, const void *d_wgt, int precision,
+                                long npix, const int64_t *d_pix, void *d_out, int sign, void *stream);
+
+/* Text of the last error on the calling thread. */
+const char *curig_last_error(void);
+
+/* Host-only plan computation (no device access): for nrow_valid unflagged rows
+ * whose w (metres) spans [wmin, wmax].  ints[8] = kw, nf1, nf2, nf3 (w planes),
+ * planes per batch, batches, tile width, tile height; reals[8] = beta, c, gamma,
+ * h, i_center, o_center, pixelsize_x, pixelsize_y. */
+int curig_plan_info(int nrow_valid, int nxdirty, int nydirty, double fov, double freq, double epsilon,
+                    double sigma, int sign, double wmin, double wmax, int precision, int *ints, double *reals);
+/* Host-only: correction factors phihat(k) (-1)^k, k = 0..nf/2, of the kernel
+ * normalised to peak 1 (multiply by e^beta for the reference's unnormalised kernel). */
+int curig_kernel_fseries(double epsilon, double sigma, int nf, double *out);
+/* Host-only: n-point Gauss-Legendre rule on [-1, 1]. */
+int curig_gauss_legendre(int n, double *x, double *w);
+
+/* Tile width / height and w planes per batch used by later calls (0 = default).
+ * The autotuner (curig_amd.autotune) sets these. */
+int curig_set_tuning(int tile_x, int tile_y, int planes_per_batch);
+/* 1: time the stages of later calls with hipEvents (adds synchronisation). */
+int curig_enable_stats(int on);
+/* Stats of the last call on this process: ints[8] = kw, nf1, nf2, nf3, planes
+ * per batch, batches, tile width, tile height; reals[8] = rows gridded, t_prep,
+ * t_grid (spread / interp), t_fft, t_plane (w accumulate / w fill), t_final,
+ * t_total (ms), timed flag. */
+int curig_get_stats(int *ints, double *reals);
+
+/* Frees cached device buffers and rocFFT plans. */
+void curig_release(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,287 @@
+"""GPU parity: the HIP gridder, called through the C ABI, against the oracle.
+
+Two oracles, two bars:
+* the numpy restatement of the reference's algorithm (oracle/nufft_ref.py):
+  the same discrete operator, so FP64 results agree to ~1e-10 relative;
+* the FP64 direct-sum DFT (oracle/dft.c): agreement to the requested epsilon
+  (3 eps + 1e-11 for FP64; FP32-mixed adds 3e-6 of single-precision roundoff).
+Adjointness is asserted at rtol 1e-12 in FP64 as the reference does
+(python/curagridder/test_wst.py:77)."""
+import sys
+import zlib
+
+import numpy as np
+import pytest
+
+from oracle import nufft_ref as R
+from tests import oracle_helpers as O
+
+pytestmark = pytest.mark.gpu
+C = 299792458.0
+
+
+def _lib():
+    from curig_amd import _lib as L
+
+    return L.load()
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rand_case(seed, nrow, nx, scale=1.0, f0=1e9, wscale=1.0):
+    rng = np.random.default_rng(seed)
+    uvw = (rng.random((nrow, 3)) - 0.5) / (f0 / C) * scale * nx / 2
+    uvw[:, 2] *= wscale
+    vis = rng.standard_normal(nrow) + 1j * rng.standard_normal(nrow)
+    wgt = rng.uniform(0.5, 1.0, nrow)
+    return uvw, vis, wgt, f0, rng
+
+
+def gpu_ms2dirty(uvw, vis, wgt, nx, ny, fov, freq, eps, sigma=2.0, sign=-1):
+    lib = _lib()
+    uvw = np.ascontiguousarray(uvw, dtype=np.float64)
+    vis = np.ascontiguousarray(vis, dtype=np.complex128)
+    dirty = np.zeros((nx, ny), dtype=np.complex128)
+    if wgt is None:
+        code = lib.ms2dirty_1(uvw.shape[0], nx, ny, fov, freq, uvw, vis, dirty, eps, sigma, sign)
+    else:
+        code = lib.ms2dirty_2(uvw.shape[0], nx, ny, fov, freq, uvw, vis, np.ascontiguousarray(wgt, np.float64),
+                              dirty, eps, sigma, sign)
+    assert code == 0, lib.curig_last_error()
+    return dirty
+
+
+def gpu_dirty2ms(uvw, img, wgt, nx, ny, fov, freq, eps, sigma=2.0, sign=-1):
+    lib = _lib()
+    uvw = np.ascontiguousarray(uvw, dtype=np.float64)
+    vis = np.zeros(uvw.shape[0], dtype=np.complex128)
+    img = np.ascontiguousarray(img, dtype=np.complex128)
+    if wgt is None:
+        code = lib.dirty2ms_1(uvw.shape[0], nx, ny, fov, freq, uvw, vis, img, eps, sigma, sign)
+    else:
+        code = lib.dirty2ms_2(uvw.shape[0], nx, ny, fov, freq, uvw, vis, np.ascontiguousarray(wgt, np.float64),
+                              img, eps, sigma, sign)
+    assert code == 0, lib.curig_last_error()
+    return vis
+
+
+def gpu_f32(to_image, uvw, data, wgt, nx, ny, fov, freq, eps, sign=-1, sigma=2.0):
+    lib = _lib()
+    uvw = np.ascontiguousarray(uvw, dtype=np.float64)
+    w = None if wgt is None else np.ascontiguousarray(wgt, np.float32)
+    wp = None if w is None else w.ctypes.data
+    if to_image:
+        vis = np.ascontiguousarray(data, dtype=np.complex64)
+        dirty = np.zeros((nx, ny), dtype=np.complex64)
+        code = lib.ms2dirty_f32(uvw.shape[0], nx, ny, fov, freq, uvw.ctypes.data, vis.ctypes.data, wp,
+                                dirty.ctypes.data, eps, sigma, sign)
+        assert code == 0, lib.curig_last_error()
+        return dirty
+    img = np.ascontiguousarray(data, dtype=np.complex64)
+    vis = np.zeros(uvw.shape[0], dtype=np.complex64)
+    code = lib.dirty2ms_f32(uvw.shape[0], nx, ny, fov, freq, uvw.ctypes.data, vis.ctypes.data, wp, img.ctypes.data,
+                            eps, sigma, sign)
+    assert code == 0, lib.curig_last_error()
+    return vis
+
+
+CASES = [
+    # name, nrow, nx, ny, fov, eps, uv scale (x Nyquist), w scale, sign, use_wgt, sigma
+    ("square_1e-6", 300, 64, 64, 10.0, 1e-6, 1.0, 1.0, -1, True, 2.0),
+    ("rect_1e-12", 200, 48, 80, 5.0, 1e-12, 1.0, 3.0, -1, False, 2.0),
+    ("wrap_2.9pi", 400, 40, 40, 20.0, 1e-4, 2.9, 1.0, 1, True, 2.0),
+    ("fold_4.7pi", 250, 36, 24, 2.0, 1e-5, 4.7, 20.0, -1, True, 2.0),
+    ("odd_sizes", 180, 31, 17, 8.0, 1e-6, 1.5, 1.0, -1, True, 2.0),
+    ("wide_w", 300, 64, 48, 30.0, 1e-7, 1.0, 40.0, -1, True, 2.0),
+    ("eps_0.2", 120, 32, 32, 10.0, 2e-1, 1.0, 1.0, -1, False, 2.0),
+    ("sigma_1.25", 150, 40, 40, 6.0, 1e-6, 1.0, 2.0, -1, True, 1.25),
+    ("big_tiles", 2000, 256, 192, 3.0, 1e-6, 1.3, 5.0, -1, True, 2.0),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_fp64_parity(case):
+    name, nrow, nx, ny, fov, eps, scale, wsc, sign, use_wgt, sigma = case
+    say(f"[fp64 parity] {name}: nrow={nrow} image={nx}x{ny} fov={fov} eps={eps} sigma={sigma}")
+    uvw, vis, wgt, f0, rng = rand_case(zlib.crc32(name.encode()), nrow, nx, scale, wscale=wsc)
+    wgt = wgt if use_wgt else None
+    img = rng.standard_normal((nx, ny)) + 1j * rng.standard_normal((nx, ny))
+    d_gpu = gpu_ms2dirty(uvw, vis, wgt, nx, ny, fov, f0, eps, sigma, sign)
+    d_ref = R.ms2dirty(uvw, vis, wgt, nx, ny, fov, f0, eps, sigma, sign)
+    d_dft = O.dft_ms2dirty(uvw, vis, wgt, nx, ny, fov, f0, sign)
+    assert O.l2error(d_ref, d_gpu) < 1e-10, "vs restatement of the reference algorithm"
+    assert O.l2error(d_dft, d_gpu) < 3 * eps + 1e-11, "vs direct sum"
+    v_gpu = gpu_dirty2ms(uvw, img, wgt, nx, ny, fov, f0, eps, sigma, sign)
+    v_ref = R.dirty2ms(uvw, img, wgt, nx, ny, fov, f0, eps, sigma, sign)
+    v_dft = O.dft_dirty2ms(uvw, img, wgt, nx, ny, fov, f0, sign)
+    assert O.l2error(v_ref, v_gpu) < 1e-10
+    assert O.l2error(v_dft, v_gpu) < 3 * eps + 1e-11
+    np.testing.assert_allclose(np.vdot(vis, v_gpu), np.vdot(d_gpu, img), rtol=1e-12)
+
+
+@pytest.mark.parametrize("case", CASES[:6], ids=[c[0] for c in CASES[:6]])
+def test_fp32_mixed_vs_dft(case):
+    name, nrow, nx, ny, fov, eps, scale, wsc, sign, use_wgt, sigma = case
+    eps = max(eps, 1e-6)
+    say(f"[fp32 parity] {name}: nrow={nrow} image={nx}x{ny} eps={eps}")
+    uvw, vis, wgt, f0, rng = rand_case(zlib.crc32(name.encode()) + 1, nrow, nx, scale, wscale=wsc)
+    wgt = wgt.astype(np.float32) if use_wgt else None
+    vis = vis.astype(np.complex64)
+    img = (rng.standard_normal((nx, ny)) + 1j * rng.standard_normal((nx, ny))).astype(np.complex64)
+    d = gpu_f32(True, uvw, vis, wgt, nx, ny, fov, f0, eps, sign, sigma)
+    d_dft = O.dft_ms2dirty(uvw, vis, wgt, nx, ny, fov, f0, sign)
+    assert O.l2error(d_dft, d) < 3 * eps + 3e-6
+    v = gpu_f32(False, uvw, img, wgt, nx, ny, fov, f0, eps, sign, sigma)
+    v_dft = O.dft_dirty2ms(uvw, img, wgt, nx, ny, fov, f0, sign)
+    assert O.l2error(v_dft, v) < 3 * eps + 3e-6
+    lhs = np.vdot(vis.astype(np.complex128), v.astype(np.complex128))
+    rhs = np.vdot(d.astype(np.complex128), img.astype(np.complex128))
+    assert abs(lhs - rhs) / abs(lhs) < 1e-5
+
+
+# the reference's own test grid (python/curagridder/test_wst.py:36-83) through the
+# NIFTY-style API, with the adjointness it asserts and the DFT error it prints
+REF_GRID = [(nrow, nx, ny, fov, eps, w)
+            for nrow in (2, 27, 100) for (nx, ny) in ((30, 128), (128, 250), (16, 64))
+            for fov in (1, 10, 20) for eps in (2e-1, 5e-3, 5e-5, 5e-7, 5e-12) for w in (False, True)]
+
+
+def test_reference_test_grid_nifty_api():
+    import curig_amd
+
+    say(f"[reference grid] {len(REF_GRID)} cases")
+    worst = 0.0
+    for (nrow, nx, ny, fov, eps, use_wgt) in REF_GRID:
+        xpixsize = fov * np.pi / 180 / nx
+        ypixsize = fov * np.pi / 180 / ny
+        rng = np.random.default_rng(42)
+        f0 = 1e9
+        freq = f0 + np.arange(1) * f0
+        uvw = (rng.random((nrow, 3)) - 0.5) / (f0 / C)
+        ms = rng.random((nrow, 1)) - 0.5 + 1j * (rng.random((nrow, 1)) - 0.5)
+        dirty = rng.random((nx, ny)) - 0.5
+        wgt = rng.random((nrow, 1)) if use_wgt else None
+        d2 = curig_amd.ms2dirty(uvw, freq, ms, wgt, nx, ny, xpixsize, ypixsize, 0, 0, eps, True)
+        ms2 = curig_amd.dirty2ms(uvw, freq, dirty, wgt, xpixsize, ypixsize, 0, 0, eps, True)
+        np.testing.assert_allclose(np.vdot(ms, ms2).real, np.vdot(d2, dirty), rtol=1e-12)
+        truth = O.dft_ms2dirty(uvw, ms[:, 0], None if wgt is None else wgt[:, 0], nx, ny, fov, f0, -1).real
+        err = O.l2error(truth, d2)
+        worst = max(worst, err / max(eps, 1e-11))
+        assert err < 3 * eps + 1e-10, (nrow, nx, ny, fov, eps, use_wgt, err)
+    say(f"[reference grid] worst error / eps = {worst:.3f}")
+
+
+def test_nrow_zero_and_single_row():
+    say("[edge] nrow=0, nrow=1 plane wave, uvw=0")
+    nx, ny, fov, f0 = 32, 24, 5.0, 1.2e9
+    d = gpu_ms2dirty(np.zeros((0, 3)), np.zeros(0, np.complex128), None, nx, ny, fov, f0, 1e-6)
+    assert np.all(d == 0)
+    v = gpu_dirty2ms(np.zeros((0, 3)), np.ones((nx, ny)), None, nx, ny, fov, f0, 1e-6)
+    assert v.shape == (0,)
+    uvw = np.array([[123.0, -45.0, 67.0]])
+    vis = np.array([0.5 - 2.0j])
+    d = gpu_ms2dirty(uvw, vis, np.array([0.8]), nx, ny, fov, f0, 1e-9)
+    i = np.arange(nx)[:, None]
+    k = np.arange(ny)[None, :]
+    l = (i - nx // 2) * fov * np.pi / 180 / nx
+    m = (k - ny // 2) * fov * np.pi / 180 / ny
+    n = np.sqrt(1 - l * l - m * m)
+    want = 0.8 * vis[0] * np.exp(2j * np.pi * f0 / C * (l * 123.0 - m * 45.0 - 67.0 * (n - 1))) / n
+    assert O.l2error(want, d) < 3e-9
+    d0 = gpu_ms2dirty(np.zeros((5, 3)), np.arange(5) * (1 + 1j), None, nx, ny, fov, f0, 1e-6)
+    assert O.l2error(10 * (1 + 1j) / n, d0) < 3e-6
+
+
+def test_row_permutation_invariance():
+    say("[edge] permutation invariance")
+    uvw, vis, wgt, f0, rng = rand_case(5, 3000, 128, 2.0, wscale=10.0)
+    nx = ny = 128
+    d0 = gpu_ms2dirty(uvw, vis, wgt, nx, ny, 4.0, f0, 1e-7)
+    p = rng.permutation(len(vis))
+    d1 = gpu_ms2dirty(uvw[p], vis[p], wgt[p], nx, ny, 4.0, f0, 1e-7)
+    assert O.l2error(d0, d1) < 1e-12
+
+
+def test_error_codes_on_device_path():
+    say("[edge] error codes")
+    lib = _lib()
+    nx = ny = 16
+    uvw = np.array([[1.0, 2.0, np.nan]])
+    vis = np.ones(1, np.complex128)
+    dirty = np.zeros((nx, ny), np.complex128)
+    assert lib.ms2dirty_1(1, nx, ny, 2.0, 1e9, uvw, vis, dirty, 1e-6, 2.0, -1) == 9
+    assert lib.ms2dirty_1(1, nx, ny, 2.0, 1e9, np.zeros((1, 3)), vis, dirty, 1e-6, 2.0, 3) == 5
+    assert lib.ms2dirty_1(1, nx, ny, 200.0, 1e9, np.zeros((1, 3)), vis, dirty, 1e-6, 2.0, -1) == 5
+
+
+def _torch():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def test_explicit_gpu_dft_vs_cpu_oracle():
+    torch = _torch()
+    say("[explicit] GPU direct sum vs CPU oracle")
+    lib = _lib()
+    uvw, vis, wgt, f0, rng = rand_case(9, 700, 40, 3.0, wscale=30.0)
+    nx, ny, fov = 40, 56, 7.0
+    pix = np.ascontiguousarray(rng.choice(nx * ny, 300, replace=False).astype(np.int64))
+    d_uvw = torch.from_numpy(uvw).cuda()
+    d_vis = torch.from_numpy(vis).cuda()
+    d_wgt = torch.from_numpy(wgt).cuda()
+    d_pix = torch.from_numpy(pix).cuda()
+    out = torch.zeros(300, dtype=torch.complex128, device="cuda")
+    code = lib.curig_explicit_ms2dirty_dev(700, nx, ny, fov, f0, d_uvw.data_ptr(), d_vis.data_ptr(),
+                                           d_wgt.data_ptr(), 0, 300, d_pix.data_ptr(), out.data_ptr(), -1, None)
+    assert code == 0
+    torch.cuda.synchronize()
+    want = O.dft_ms2dirty_pixels(uvw, vis, wgt, nx, ny, fov, f0, pix, -1)
+    np.testing.assert_allclose(out.cpu().numpy(), want, rtol=0, atol=1e-11 * np.abs(want).max())
+
+
+def test_multichannel_flags_vs_per_channel():
+    torch = _torch()
+    say("[multi] 3 channels with flags, summed and per-channel images")
+    lib = _lib()
+    nrow, nchan, nx, ny, fov = 900, 3, 64, 64, 5.0
+    rng = np.random.default_rng(21)
+    uvw = (rng.random((nrow, 3)) - 0.5) / (1e9 / C) * 40
+    freq = np.array([1.0e9, 1.2e9, 1.5e9])
+    vis = rng.standard_normal((nrow, nchan)) + 1j * rng.standard_normal((nrow, nchan))
+    wgt = rng.uniform(0.5, 1, (nrow, nchan))
+    flag = (rng.random((nrow, nchan)) < 0.3).astype(np.uint8)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    d_uvw, d_vis, d_wgt, d_flag = t(uvw), t(vis), t(wgt), t(flag)
+    per = torch.zeros((nchan, nx, ny), dtype=torch.complex128, device="cuda")
+    tot = torch.zeros((nx, ny), dtype=torch.complex128, device="cuda")
+    for out, pc in ((per, 1), (tot, 0)):
+        code = lib.curig_ms2dirty_multi_dev(nrow, nchan, nx, ny, fov, freq, d_uvw.data_ptr(), d_vis.data_ptr(),
+                                            d_wgt.data_ptr(), d_flag.data_ptr(), out.data_ptr(), pc, 1e-8, 2.0, -1,
+                                            0, None)
+        assert code == 0
+    torch.cuda.synchronize()
+    per_h = per.cpu().numpy()
+    for c in range(nchan):
+        keep = flag[:, c] == 0
+        want = O.dft_ms2dirty(uvw[keep], vis[keep, c], wgt[keep, c], nx, ny, fov, freq[c], -1)
+        assert O.l2error(want, per_h[c]) < 3e-8
+    assert O.l2error(per_h.sum(0), tot.cpu().numpy()) < 1e-13
+    img = rng.standard_normal((nx, ny)) + 1j * rng.standard_normal((nx, ny))
+    d_img = t(img)
+    vout = torch.zeros((nrow, nchan), dtype=torch.complex128, device="cuda")
+    code = lib.curig_dirty2ms_multi_dev(nrow, nchan, nx, ny, fov, freq, d_uvw.data_ptr(), vout.data_ptr(),
+                                        d_wgt.data_ptr(), d_flag.data_ptr(), d_img.data_ptr(), 0, 1e-8, 2.0, -1, 0,
+                                        None)
+    assert code == 0
+    torch.cuda.synchronize()
+    vh = vout.cpu().numpy()
+    for c in range(nchan):
+        keep = flag[:, c] == 0
+        want = O.dft_dirty2ms(uvw[keep], img, wgt[keep, c], nx, ny, fov, freq[c], -1)
+        assert O.l2error(want, vh[keep, c]) < 3e-8
+        assert np.all(vh[~keep, c] == 0)

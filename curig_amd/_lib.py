@@ -41,6 +41,14 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"libcurafft.so not built ({LIB_PATH}); run python -m curig_amd.build")
+    # One HIP runtime per process: PyTorch-ROCm ships its own libamdhip64.so.7 and
+    # the dynamic loader binds every later user of that soname to the first copy
+    # loaded, so torch (when installed) is imported first and this library shares
+    # its runtime; device pointers then pass freely between the two.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.cdll.LoadLibrary(LIB_PATH)
     ref_args = [c_int, c_int, c_int, c_double, c_double, _f64, _c128, _c128, c_double, c_double, c_int]
     ref_args_w = [c_int, c_int, c_int, c_double, c_double, _f64, _c128, _f64, _c128, c_double, c_double, c_int]

@@ -762,18 +762,18 @@ struct Timer {
     if (!on) return -1;
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return -1;
-    hipEventRecord(e, s);
+    (void)hipEventRecord(e, s);
     ev.push_back(e);
     return (int)ev.size() - 1;
   }
   double ms(int a, int b) {
     if (!on || a < 0 || b < 0) return 0;
     float t = 0;
-    hipEventElapsedTime(&t, ev[a], ev[b]);
+    (void)hipEventElapsedTime(&t, ev[a], ev[b]);
     return t;
   }
   ~Timer() {
-    for (auto e : ev) hipEventDestroy(e);
+    for (auto e : ev) (void)hipEventDestroy(e);
   }
 };
 
@@ -839,6 +839,10 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
   g.D = o_min / 2.0;
   double S = std::fabs(o_min / 2.0);
   set_nhg_type3(S, X, ko, g.P, g.h, g.gamma);
+  if (g.P > 10000000) {
+    set_error("w range needs %d w planes (more than 1e7)", g.P);
+    return kErrSize;
+  }
   g.nf1 = get_num_cells(ny, ko);
   g.nf2 = get_num_cells(nx, ko);
   if ((double)g.nf1 * g.nf2 > 2.0e9) {
@@ -986,15 +990,24 @@ void interp_dispatch(const Rec<T> *recs, const v2_t<T> *grid, double2 *vacc, con
     interp_kernel<T, kMaxKernelWidth><<<nbk, 256, 0, st>>>(recs, grid, vacc, g, p0, nb, ra0, ra1, rb0, rb1);
 }
 
-// FFT for a batch of nb planes (full batches share one plan, the tail has its own,
-// like fftplan / fftplan_l in the reference, cugridder.cu:230-246).
+// FFT for a batch of nb planes, at most kMaxFftElems elements per rocFFT call
+// (the reference caps cuFFT batches the same way, MAX_CUFFT_ELEM, cugridder.cu:230-246;
+// a single call over 2^32 elements returns wrong planes).
+constexpr size_t kMaxFftElems = size_t(1) << 28;
+
 template <typename T>
 int batch_fft(int dev, const Geom &g, int nb, int sign, void *grid, hipStream_t st) {
-  FftPlan fp;
-  CURIG_TRY(fft_plan(dev, g.nf1, g.nf2, nb, sizeof(T) == 8, sign, &fp));
-  void *work = nullptr;
-  if (fp.work_bytes) CURIG_TRY(ws_get(dev, S_FFTW, fp.work_bytes, &work));
-  return fft_exec(fp, grid, work, st);
+  const size_t plane = (size_t)g.nf1 * g.nf2;
+  const int chunk = (int)std::max<size_t>(1, std::min<size_t>(nb, kMaxFftElems / plane));
+  for (int k0 = 0; k0 < nb; k0 += chunk) {
+    int n = std::min(chunk, nb - k0);
+    FftPlan fp;
+    CURIG_TRY(fft_plan(dev, g.nf1, g.nf2, n, sizeof(T) == 8, sign, &fp));
+    void *work = nullptr;
+    if (fp.work_bytes) CURIG_TRY(ws_get(dev, S_FFTW, fp.work_bytes, &work));
+    CURIG_TRY(fft_exec(fp, (v2_t<T> *)grid + (size_t)k0 * plane, work, st));
+  }
+  return kOk;
 }
 
 // rows whose first plane lies in [lo, hi] (cyclic, length <= P) -> up to two record ranges

@@ -92,7 +92,14 @@ void set_nhg_type3(double S, double X, const KernelOpts &o, int &nf, double &h, 
   }
   double nfd = 2.0 * o.upsampfac * Ssafe * Xsafe / M_PI + nss;
   if (!std::isfinite(nfd)) nfd = 0.0;
+  if (nfd > 2.0e9) nfd = 2.0e9;  // rejected by the caller (too many w planes)
   nf = (int)nfd;
+  if (nf >= 1000000000) {
+    nf = 1000000000;
+    h = 2 * M_PI / nf;
+    gam = (double)nf / (2.0 * o.upsampfac * Ssafe);
+    return;
+  }
   if (nf < 2 * o.kw) nf = 2 * o.kw;
   nf = next235beven(nf, 1);
   h = 2 * M_PI / nf;

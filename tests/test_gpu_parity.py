@@ -285,3 +285,73 @@ def test_multichannel_flags_vs_per_channel():
         want = O.dft_dirty2ms(uvw[keep], img, wgt[keep, c], nx, ny, fov, freq[c], -1)
         assert O.l2error(want, vh[keep, c]) < 3e-8
         assert np.all(vh[~keep, c] == 0)
+
+
+def _c3_like_rows(nrow, seed=4):
+    import bench
+
+    uvw = bench.make_uvw(256, 20000.0, 768, 4.0, seed=1)
+    rng = np.random.default_rng(seed)
+    return uvw[rng.choice(len(uvw), nrow, replace=False)]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_long_baseline_many_planes(precision):
+    """The long-baseline geometry (20 km array, fov 6 deg, 1.4 GHz: ~1800 w planes,
+    dozens of plane batches, |x| folded over ~150 periods) on a 256^2 image,
+    against the direct sum at 3000 pixels."""
+    nrow, nx, ny, fov, f0, eps = 30000, 256, 256, 6.0, 1.4e9, 1e-5
+    say(f"[long baseline] {precision}: nrow={nrow} image={nx}x{ny} fov={fov} eps={eps}")
+    uvw = _c3_like_rows(nrow)
+    rng = np.random.default_rng(11)
+    vis = rng.standard_normal(nrow) + 1j * rng.standard_normal(nrow)
+    wgt = rng.uniform(0.5, 1, nrow)
+    pix = rng.choice(nx * ny, 3000, replace=False)
+    truth = O.dft_ms2dirty_pixels(uvw, vis, wgt, nx, ny, fov, f0, pix, -1)
+    if precision == "fp64":
+        d = gpu_ms2dirty(uvw, vis, wgt, nx, ny, fov, f0, eps)
+        tol = 3 * eps
+    else:
+        d = gpu_f32(True, uvw, vis, wgt, nx, ny, fov, f0, eps)
+        tol = 3 * eps + 3e-6
+    from curig_amd import _lib as L
+
+    info = L.plan_info(nrow, nx, ny, fov, f0, eps, 2.0, -1, uvw[:, 2].min(), uvw[:, 2].max(),
+                       0 if precision == "fp64" else 1)
+    err = O.l2error(truth, d.reshape(-1)[pix])
+    say(f"[long baseline] planes={info['nf3']} batches={info['nbatch']} err={err:.3e}")
+    assert info["nf3"] > 1000 and info["nbatch"] > 10
+    assert err < tol
+    # dirty2ms at 300 rows against the direct sum
+    img = rng.standard_normal((nx, ny)) + 1j * rng.standard_normal((nx, ny))
+    sel = rng.choice(nrow, 300, replace=False)
+    if precision == "fp64":
+        v = gpu_dirty2ms(uvw, img, wgt, nx, ny, fov, f0, eps)
+    else:
+        v = gpu_f32(False, uvw, img, wgt, nx, ny, fov, f0, eps)
+    vt = O.dft_dirty2ms(uvw[sel], img, wgt[sel], nx, ny, fov, f0, -1)
+    verr = O.l2error(vt, v[sel])
+    say(f"[long baseline] dirty2ms err={verr:.3e}")
+    assert verr < tol
+
+
+@pytest.mark.parametrize("planes_per_batch", [1, 3, 5])
+def test_small_plane_batches_match_restatement(planes_per_batch):
+    """Plane-batch boundaries (spread source batches, interpolation ranges, the
+    w-accumulate across batches and the cyclic plane index) against the
+    restatement with every batch size."""
+    lib = _lib()
+    say(f"[plane batches] {planes_per_batch}")
+    assert lib.curig_set_tuning(0, 0, planes_per_batch) == 0
+    try:
+        uvw, vis, wgt, f0, rng = rand_case(77, 400, 48, 1.7, wscale=25.0)
+        nx, ny, fov, eps = 48, 40, 12.0, 1e-7
+        img = rng.standard_normal((nx, ny)) + 1j * rng.standard_normal((nx, ny))
+        d = gpu_ms2dirty(uvw, vis, wgt, nx, ny, fov, f0, eps)
+        assert O.l2error(R.ms2dirty(uvw, vis, wgt, nx, ny, fov, f0, eps), d) < 1e-10
+        v = gpu_dirty2ms(uvw, img, wgt, nx, ny, fov, f0, eps)
+        assert O.l2error(R.dirty2ms(uvw, img, wgt, nx, ny, fov, f0, eps), v) < 1e-10
+        d32 = gpu_f32(True, uvw, vis, wgt, nx, ny, fov, f0, eps)
+        assert O.l2error(R.ms2dirty(uvw, vis, wgt, nx, ny, fov, f0, eps), d32) < 3e-6
+    finally:
+        lib.curig_set_tuning(0, 0, 0)

@@ -21,7 +21,7 @@ EXPORTS = [
     "ms2dirty_f32", "dirty2ms_f32", "curig_ms2dirty_dev", "curig_dirty2ms_dev",
     "curig_ms2dirty_multi_dev", "curig_dirty2ms_multi_dev", "curig_explicit_ms2dirty_dev",
     "curig_last_error", "curig_plan_info", "curig_kernel_fseries", "curig_gauss_legendre",
-    "curig_set_tuning", "curig_enable_stats", "curig_get_stats", "curig_release",
+    "curig_set_tuning", "curig_set_fft_path", "curig_enable_stats", "curig_get_stats", "curig_release",
 ]
 # C++-linkage plan API (include/cugridder.h), exported with mangled names
 CXX_EXPORTS = ["gridder_setting", "gridder_execution", "gridder_destroy"]
@@ -89,6 +89,8 @@ def load():
     lib.curig_gauss_legendre.restype = c_int
     lib.curig_set_tuning.argtypes = [c_int, c_int, c_int]
     lib.curig_set_tuning.restype = c_int
+    lib.curig_set_fft_path.argtypes = [c_int]
+    lib.curig_set_fft_path.restype = c_int
     lib.curig_enable_stats.argtypes = [c_int]
     lib.curig_enable_stats.restype = c_int
     lib.curig_get_stats.argtypes = [np.ctypeslib.ndpointer(np.int32, flags="C"), _f64]
@@ -107,10 +109,10 @@ def check(code):
 
 
 def plan_info(nrow_valid, nx, ny, fov, freq, eps, sigma, sign, wmin, wmax, precision=0):
-    ints = np.zeros(8, dtype=np.int32)
+    ints = np.zeros(9, dtype=np.int32)
     reals = np.zeros(8, dtype=np.float64)
     check(load().curig_plan_info(nrow_valid, nx, ny, fov, freq, eps, sigma, sign, wmin, wmax, precision, ints, reals))
-    keys_i = ["kw", "nf1", "nf2", "nf3", "planes_per_batch", "nbatch", "tile_x", "tile_y"]
+    keys_i = ["kw", "nf1", "nf2", "nf3", "planes_per_batch", "nbatch", "tile_x", "tile_y", "fft_lds"]
     keys_r = ["beta", "c", "gamma", "h", "i_center", "o_center", "pixelsize_x", "pixelsize_y"]
     d = {k: int(v) for k, v in zip(keys_i, ints)}
     d.update({k: float(v) for k, v in zip(keys_r, reals)})
@@ -118,10 +120,10 @@ def plan_info(nrow_valid, nx, ny, fov, freq, eps, sigma, sign, wmin, wmax, preci
 
 
 def stats():
-    ints = np.zeros(8, dtype=np.int32)
+    ints = np.zeros(9, dtype=np.int32)
     reals = np.zeros(8, dtype=np.float64)
     load().curig_get_stats(ints, reals)
-    keys_i = ["kw", "nf1", "nf2", "nf3", "planes_per_batch", "nbatch", "tile_x", "tile_y"]
+    keys_i = ["kw", "nf1", "nf2", "nf3", "planes_per_batch", "nbatch", "tile_x", "tile_y", "fft_lds"]
     keys_r = ["nvis", "t_prep", "t_grid", "t_fft", "t_plane", "t_final", "t_total", "timed"]
     d = {k: int(v) for k, v in zip(keys_i, ints)}
     d.update({k: float(v) for k, v in zip(keys_r, reals)})

@@ -260,6 +260,7 @@ static int multi(bool to_image, int nrow, int nchan, int nx, int ny, double fov,
     g_last_stats.nbatch = last.nbatch;
     g_last_stats.T1 = last.T1;
     g_last_stats.T2 = last.T2;
+    g_last_stats.fft_lds = last.fft_lds;
     g_last_stats.timed = 1;
   }
   return kOk;
@@ -308,7 +309,7 @@ extern "C" int curig_plan_info(int nrow_valid, int nxdirty, int nydirty, double 
   CURIG_TRY(plan_geometry(nrow_valid, nxdirty, nydirty, fov, freq, epsilon, sigma, sign, wmin, wmax, precision == 0,
                           &g_tuning, &g, &ko));
   if (ints) {
-    int v[8] = {g.kw, g.nf1, g.nf2, g.P, g.Pb, g.nbatch, g.T1, g.T2};
+    int v[9] = {g.kw, g.nf1, g.nf2, g.P, g.Pb, g.nbatch, g.T1, g.T2, g.fft_lds};
     memcpy(ints, v, sizeof(v));
   }
   if (reals) {
@@ -351,6 +352,15 @@ extern "C" int curig_set_tuning(int tile_x, int tile_y, int planes_per_batch) {
   return kOk;
 }
 
+extern "C" int curig_set_fft_path(int path) {
+  if (path != 0 && path != 1) {
+    set_error("fft path must be 0 (auto) or 1 (rocFFT)");
+    return kErrArg;
+  }
+  g_tuning.fft = path;
+  return kOk;
+}
+
 extern "C" int curig_enable_stats(int on) {
   g_stats_on = on != 0;
   return kOk;
@@ -359,7 +369,7 @@ extern "C" int curig_enable_stats(int on) {
 extern "C" int curig_get_stats(int *ints, double *reals) {
   const Stats &s = g_last_stats;
   if (ints) {
-    int v[8] = {s.kw, s.nf1, s.nf2, s.P, s.Pb, s.nbatch, s.T1, s.T2};
+    int v[9] = {s.kw, s.nf1, s.nf2, s.P, s.Pb, s.nbatch, s.T1, s.T2, s.fft_lds};
     memcpy(ints, v, sizeof(v));
   }
   if (reals) {

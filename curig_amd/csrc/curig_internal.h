@@ -81,14 +81,25 @@ struct Geom {
   double uscale, vscale, wscale;  // 2 pi f/c dx, 2 pi f/c dy, 2 pi f/c sign
   int T1, T2, ntx, nty, ntiles;   // tile size (cells) along nf1 / nf2, tile counts
   int Pb, nbatch;     // planes per batch, number of batches
+  int Pbin, nlayer;   // planes per bin layer (rows are binned by first plane / Pbin), layers
   int nquad;          // number of quadrature nodes for phihat3
+  // pruned LDS FFT path (power-of-two grids); fft_lds == 0 selects rocFFT
+  int fft_lds;
+  int A2, B2;         // nf2 = A2 * B2 (four-step split of the column transform)
+  int Wc1, Wc2;       // columns per workgroup in the two column passes
+  int rpw;            // rows per workgroup in the row pass
+  int Erow;           // elements per thread in the row pass (16)
+  int lds_row_bytes;  // dynamic LDS of the row-pass workgroup
 };
 
 // Tunables (autotuned per problem class; defaults are safe for any size)
 struct Tuning {
-  int T1 = 0, T2 = 0;   // 0 -> default for the precision
+  int T1 = 0, T2 = 0;   // 0 -> default for the precision (tile / row-strip size)
+  int Pbin = 0;         // planes per bin layer (0 -> 8)
   int Pb = 0;           // 0 -> from the memory budget
-  size_t grid_budget_bytes = size_t(32) << 30;
+  int fft = 0;          // 0 auto (LDS path when it applies), 1 force rocFFT
+  int lds_bytes = 0;    // LDS per workgroup of the device (0: 160 KiB, gfx950)
+  size_t grid_budget_bytes = size_t(24) << 30;
 };
 
 }  // namespace curig

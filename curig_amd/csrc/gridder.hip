@@ -28,6 +28,7 @@
 
 #include "curig_internal.h"
 #include "fft.h"
+#include "fft_core.h"
 #include "gridder.h"
 
 namespace curig {
@@ -119,7 +120,7 @@ __device__ __forceinline__ Coords row_coords(const double *__restrict__ uvw, lon
 
 __device__ __forceinline__ int row_bin(const Coords &c, const Geom &g) {
   int tile = (c.iy / g.T2) * g.ntx + (c.ix / g.T1);
-  return (c.iw / g.Pb) * g.ntiles + tile;
+  return (c.iw / g.Pbin) * g.ntiles + tile;
 }
 
 template <typename T> __device__ __forceinline__ double2 load_c(const void *p, long j);
@@ -199,6 +200,11 @@ __global__ void __launch_bounds__(256) w_minmax_kernel(const double *__restrict_
     part[2 * blockIdx.x + 1] = smax[0];
     cnt[blockIdx.x] = scnt[0];
   }
+}
+
+__global__ void gather_layers_kernel(const int *__restrict__ off, int ntiles, int nlayer, int *__restrict__ out) {
+  for (long l = blockIdx.x * (long)blockDim.x + threadIdx.x; l <= nlayer; l += (long)gridDim.x * blockDim.x)
+    out[l] = off[l * ntiles];
 }
 
 __global__ void fill_int_kernel(int *__restrict__ p, long n, int v) {
@@ -387,8 +393,7 @@ __global__ void __launch_bounds__(256) spread_tile_kernel(const Rec<T> *__restri
   const int kw = g.kw;
   const T beta = (T)g.beta, cc = (T)g.c, hw = (T)(0.5 * kw);
   // source rows: first plane in [p-kw+1, p], first cell in [x0-kw+1, x0+wx-1] x [y0-kw+1, y0+wy-1]
-  const int nbat = g.nbatch;
-  walk_tiles(imod(p - kw + 1, g.P), kw, g.P, g.Pb, nbat, [&](int sb) {
+  walk_tiles(imod(p - kw + 1, g.P), kw, g.P, g.Pbin, g.nlayer, [&](int sb) {
     walk_tiles(imod(y0 - kw + 1, g.nf2), wy + kw - 1, g.nf2, g.T2, g.nty, [&](int sty) {
       walk_tiles(imod(x0 - kw + 1, g.nf1), wx + kw - 1, g.nf1, g.T1, g.ntx, [&](int stx) {
         const int bin = sb * g.ntiles + sty * g.ntx + stx;
@@ -703,6 +708,276 @@ __global__ void __launch_bounds__(256) explicit_kernel(const double *__restrict_
   }
 }
 
+
+// --------------------------------------------------------------------------
+// pruned LDS FFT path (power-of-two grids)
+//
+// ms2dirty per plane: row pass (length nf1, keep the N1 columns of the image)
+// -> column pass 1 (length-A2 transforms over rows spaced B2 apart, twiddle)
+// -> column pass 2 (length-B2 transforms over consecutive rows, keep the N2
+// image rows) -> compact plane [N2][N1] in image order.  dirty2ms runs the
+// mirror image: compact w-filled plane -> column passes (all nf2 outputs) ->
+// row pass that expands each row to nf1 cells.  nf2 = A2 * B2; a column
+// element k2 = B2 ka + kb, an output a2 = r + A2 q (four-step FFT).
+// --------------------------------------------------------------------------
+using fftc::cpx;
+
+template <typename T, int E, int MAXR = 16>
+__device__ __forceinline__ void lds_fft(cpx<T> *buf, const fftc::Layout &L, int sign,
+                                        const cpx<T> *__restrict__ tw, int n_tw) {
+  cpx<T> v[E];
+  const int nt = blockDim.x;
+  for (int Ns = 1; Ns < L.n;) {
+    const int R = fftc::stage_radix(L.n, Ns, MAXR);
+    // opaque copy of the thread id: keeps each stage's address arithmetic inside
+    // the stage (hoisted out of the loop it is held for all stages and spills)
+    int tid = threadIdx.x;
+    __asm__ volatile("" : "+v"(tid));
+    if (R == 16)
+      fftc::stage_load<T, E, 16>(buf, v, L, Ns, sign, tw, n_tw, tid, nt);
+    else if (R == 8)
+      fftc::stage_load<T, E, 8>(buf, v, L, Ns, sign, tw, n_tw, tid, nt);
+    else if (R == 4)
+      fftc::stage_load<T, E, 4>(buf, v, L, Ns, sign, tw, n_tw, tid, nt);
+    else
+      fftc::stage_load<T, E, 2>(buf, v, L, Ns, sign, tw, n_tw, tid, nt);
+    __syncthreads();
+    if (R == 16)
+      fftc::stage_store<T, E, 16>(buf, v, L, Ns, tid, nt);
+    else if (R == 8)
+      fftc::stage_store<T, E, 8>(buf, v, L, Ns, tid, nt);
+    else if (R == 4)
+      fftc::stage_store<T, E, 4>(buf, v, L, Ns, tid, nt);
+    else
+      fftc::stage_store<T, E, 2>(buf, v, L, Ns, tid, nt);
+    __syncthreads();
+    Ns *= R;
+  }
+}
+
+__device__ __forceinline__ int image_index(int k, int N, int nf) {
+  // grid index (FFT order) -> image index (centred), or -1 outside the image
+  const int h = N / 2;
+  return k < N - h ? k + h : (k >= nf - h ? k - nf + h : -1);
+}
+__device__ __forceinline__ int grid_index(int i, int N, int nf) {
+  const int h = N / 2;
+  return i >= h ? i - h : nf + i - h;
+}
+
+// ms2dirty: spread into full grid rows held in LDS, then the row FFT (length
+// nf1), keeping the N1 image columns: H [nb][nf2][N1].  One workgroup owns rpw
+// rows of one w plane; rows are binned into full-width strips of T2 rows.
+template <typename T, int E, int KWM>
+__global__ void __launch_bounds__(1024) spread_row_fft_kernel(const Rec<T> *__restrict__ recs,
+                                                             const int *__restrict__ off, cpx<T> *__restrict__ H,
+                                                             const cpx<T> *__restrict__ tw, Geom g, int p0) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
+  T *ldsf = reinterpret_cast<T *>(smem_raw);
+  const int rpw = g.rpw, nf1 = g.nf1, nf2 = g.nf2, nt = blockDim.x;
+  const int row0 = blockIdx.x * rpw, pl = blockIdx.y, p = p0 + pl;
+  const int tot = rpw * nf1;
+  const int ntot = fftc::padded_size(tot);
+  for (int i = threadIdx.x; i < ntot; i += nt) lds[i] = fftc::mk((T)0, (T)0);
+  __syncthreads();
+  const int kw = g.kw;
+  const T beta = (T)g.beta, cc = (T)g.c, hw = (T)(0.5 * kw);
+  walk_tiles(imod(p - kw + 1, g.P), kw, g.P, g.Pbin, g.nlayer, [&](int sl) {
+    walk_tiles(imod(row0 - kw + 1, nf2), rpw + kw - 1, nf2, g.T2, g.nty, [&](int sty) {
+      const int bin = sl * g.ntiles + sty;
+      const int r0 = off[bin], r1 = off[bin + 1];
+      for (int r = r0 + threadIdx.x; r < r1; r += nt) {
+        const Rec<T> rc = recs[r];
+        int k = p - rc.iw;
+        if (k < 0) k += g.P;
+        if (k >= kw) continue;
+        int ry = rc.iy - row0;
+        if (ry < 0) ry += nf2;
+        if (!(ry < rpw || ry + kw - 1 >= nf2)) continue;
+        const T vw = es_eval(rc.dw + (T)k, beta, cc, hw);
+        const T vre = rc.re * vw, vim = rc.im * vw;
+        T kx[KWM];
+#pragma unroll
+        for (int i = 0; i < KWM; ++i) kx[i] = (i < kw) ? es_eval(rc.dx + (T)i, beta, cc, hw) : (T)0;
+        for (int jj = 0; jj < kw; ++jj) {
+          int cy = ry + jj;
+          if (cy >= nf2) cy -= nf2;
+          if (cy >= rpw) continue;
+          const T ky = es_eval(rc.dy + (T)jj, beta, cc, hw);
+          const T are = vre * ky, aim = vim * ky;
+          const int base = cy * nf1;
+#pragma unroll
+          for (int i = 0; i < KWM; ++i) {
+            if (i < kw) {
+              int cx = rc.ix + i;
+              if (cx >= nf1) cx -= nf1;
+              const int a = 2 * fftc::pad(base + cx);
+              atomicAdd(ldsf + a, are * kx[i]);
+              atomicAdd(ldsf + a + 1, aim * kx[i]);
+            }
+          }
+        }
+      }
+    });
+  });
+  __syncthreads();
+  const fftc::Layout L{nf1, rpw, nf1, 1, false};
+  lds_fft<T, E, 8>(lds, L, +1, tw, nf1);
+  cpx<T> *dst = H + ((size_t)pl * nf2 + row0) * g.N1;
+  for (int i = threadIdx.x; i < rpw * g.N1; i += nt) {
+    const int r = i / g.N1, col = i - r * g.N1;
+    dst[i] = lds[fftc::pad(r * nf1 + grid_index(col, g.N1, nf1))];
+  }
+}
+
+// dirty2ms row pass: H2 [nb][nf2][N1] -> G [nb][nf2][nf1] (zero outside the image columns)
+template <typename T, int E>
+__global__ void __launch_bounds__(1024) row_ifft_expand_kernel(const cpx<T> *__restrict__ H2,
+                                                               cpx<T> *__restrict__ G,
+                                                               const cpx<T> *__restrict__ tw, int nf1, int nf2,
+                                                               int N1, int rpw, int sign) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
+  const int row0 = blockIdx.x * rpw, pl = blockIdx.y, nt = blockDim.x;
+  const cpx<T> *src = H2 + ((size_t)pl * nf2 + row0) * N1;
+  const int tot = rpw * nf1;
+  for (int i = threadIdx.x; i < tot; i += nt) {
+    const int r = i / nf1, k1 = i - r * nf1;
+    const int col = image_index(k1, N1, nf1);
+    lds[fftc::pad(i)] = col >= 0 ? src[r * N1 + col] : fftc::mk((T)0, (T)0);
+  }
+  __syncthreads();
+  const fftc::Layout L{nf1, rpw, nf1, 1, false};
+  lds_fft<T, E, 8>(lds, L, sign, tw, nf1);
+  cpx<T> *dst = G + ((size_t)pl * nf2 + row0) * nf1;
+  for (int i = threadIdx.x; i < tot; i += nt) dst[i] = lds[fftc::pad(i)];
+}
+
+// column pass 1: length-A transforms over k2 = B ka + kb (fixed kb), then the
+// four-step twiddle w^{sign r kb}.  Source: H itself (ms2dirty, in place) or the
+// compact w-filled plane Fc [nb][N2][N1] (dirty2ms; rows outside the image are 0).
+template <typename T, bool FROM_COMPACT>
+__global__ void __launch_bounds__(256) col_pass1_kernel(const cpx<T> *src, cpx<T> *H,
+                                                         const cpx<T> *__restrict__ tw2, int nf2, int N1, int N2,
+                                                         int A, int B, int Wc, int sign) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
+  const int kb = blockIdx.x % B, cb = blockIdx.x / B, pl = blockIdx.y, nt = blockDim.x;
+  const int c0 = cb * Wc, wc = min(Wc, N1 - c0);
+  const int tot = A * Wc;
+  for (int i = threadIdx.x; i < tot; i += nt) {
+    const int ka = i / Wc, c = i - ka * Wc;
+    const int k2 = B * ka + kb;
+    cpx<T> v = fftc::mk((T)0, (T)0);
+    if (c < wc) {
+      if (FROM_COMPACT) {
+        const int row = image_index(k2, N2, nf2);
+        if (row >= 0) v = src[((size_t)pl * N2 + row) * N1 + c0 + c];
+      } else {
+        v = src[((size_t)pl * nf2 + k2) * N1 + c0 + c];
+      }
+    }
+    lds[fftc::pad(i)] = v;
+  }
+  __syncthreads();
+  const fftc::Layout L{A, Wc, 1, Wc, true};
+  lds_fft<T, 16>(lds, L, sign, tw2, nf2);
+  for (int i = threadIdx.x; i < tot; i += nt) {
+    const int r = i / Wc, c = i - r * Wc;
+    if (c < wc) {
+      const cpx<T> w = tw2[r * kb];
+      H[((size_t)pl * nf2 + B * r + kb) * N1 + c0 + c] = fftc::mulw(lds[fftc::pad(i)], w.x, (T)sign * w.y);
+    }
+  }
+}
+
+// column pass 2: length-B transforms over k2 = B r + kb (fixed r); output a2 = r + A q.
+// PRUNE (ms2dirty): keep the N2 image rows, compact [nb][N2][N1].  Otherwise all nf2 rows.
+template <typename T, bool PRUNE>
+__global__ void __launch_bounds__(256) col_pass2_kernel(const cpx<T> *__restrict__ H, cpx<T> *__restrict__ out,
+                                                         const cpx<T> *__restrict__ tw2, int nf2, int N1, int N2,
+                                                         int A, int B, int Wc, int sign) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
+  const int r = blockIdx.x % A, cb = blockIdx.x / A, pl = blockIdx.y, nt = blockDim.x;
+  const int c0 = cb * Wc, wc = min(Wc, N1 - c0);
+  const int tot = B * Wc;
+  for (int i = threadIdx.x; i < tot; i += nt) {
+    const int kb = i / Wc, c = i - kb * Wc;
+    lds[fftc::pad(i)] = c < wc ? H[((size_t)pl * nf2 + B * r + kb) * N1 + c0 + c] : fftc::mk((T)0, (T)0);
+  }
+  __syncthreads();
+  const fftc::Layout L{B, Wc, 1, Wc, true};
+  lds_fft<T, 16>(lds, L, sign, tw2, nf2);
+  for (int i = threadIdx.x; i < tot; i += nt) {
+    const int q = i / Wc, c = i - q * Wc;
+    if (c >= wc) continue;
+    const int a2 = r + A * q;
+    if (PRUNE) {
+      const int row = image_index(a2, N2, nf2);
+      if (row >= 0) out[((size_t)pl * N2 + row) * N1 + c0 + c] = lds[fftc::pad(i)];
+    } else {
+      out[((size_t)pl * nf2 + a2) * N1 + c0 + c] = lds[fftc::pad(i)];
+    }
+  }
+}
+
+// dirty2ms w-fill into compact planes: Fc[k][pix] = g[pix] e^{-i z'(p0 + k - P/2)}
+template <typename T>
+__global__ void __launch_bounds__(256) wfill_compact_kernel(const double2 *__restrict__ gimg, cpx<T> *__restrict__ Fc,
+                                                             Geom g, int p0, int nb) {
+  const long npix = (long)g.N1 * g.N2;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < npix; idx += (long)gridDim.x * blockDim.x) {
+    const int row = (int)(idx / g.N1), col = (int)(idx - (long)row * g.N1);
+    const double2 v = gimg[idx];
+    double nm1, n;
+    pixel_n(row, col, g, nm1, n);
+    const double zp = (nm1 - g.D) * g.gamma * g.h;
+    double s, c, ss, cs;
+    sincos(-zp * (double)(p0 - g.P / 2), &s, &c);
+    sincos(-zp, &ss, &cs);
+    for (int k = 0; k < nb; ++k) {
+      Fc[(size_t)k * npix + idx] = fftc::mk((T)(v.x * c - v.y * s), (T)(v.x * s + v.y * c));
+      const double cn = c * cs - s * ss;
+      s = s * cs + c * ss;
+      c = cn;
+    }
+  }
+}
+
+// w-accumulate over compact planes Fc [nb][N2][N1] (ms2dirty, LDS FFT path)
+template <typename T>
+__global__ void __launch_bounds__(256) wacc_compact_kernel(const cpx<T> *__restrict__ Fc, double2 *__restrict__ acc,
+                                                            Geom g, int p0, int nb, int first) {
+  const long npix = (long)g.N1 * g.N2;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < npix; idx += (long)gridDim.x * blockDim.x) {
+    const int row = (int)(idx / g.N1), col = (int)(idx - (long)row * g.N1);
+    double nm1, n;
+    pixel_n(row, col, g, nm1, n);
+    const double zp = (nm1 - g.D) * g.gamma * g.h;
+    double s, c, ss, cs;
+    sincos(zp * (double)(p0 - g.P / 2), &s, &c);
+    sincos(zp, &ss, &cs);
+    double ar = 0.0, ai = 0.0;
+    for (int k = 0; k < nb; ++k) {
+      const cpx<T> f = Fc[(size_t)k * npix + idx];
+      const double fr = (double)f.x, fi = (double)f.y;
+      ar += fr * c - fi * s;
+      ai += fr * s + fi * c;
+      const double cn = c * cs - s * ss;
+      s = s * cs + c * ss;
+      c = cn;
+    }
+    if (first) {
+      acc[idx] = make_double2(ar, ai);
+    } else {
+      const double2 o = acc[idx];
+      acc[idx] = make_double2(o.x + ar, o.y + ai);
+    }
+  }
+}
+
 // ==========================================================================
 // host side
 // ==========================================================================
@@ -711,11 +986,13 @@ namespace {
 // Grow-only device buffers, one set per device (calls on one device are serialised).
 enum Slot {
   S_PART, S_CNT, S_HIST, S_OFF, S_BSUM, S_BINRANK, S_RECS, S_PERM, S_GRID, S_ACC, S_VACC, S_TAB, S_FFTW,
+  S_H, S_H2, S_TW,
   S_NSLOT
 };
 struct Workspace {
   void *p[S_NSLOT] = {};
   size_t n[S_NSLOT] = {};
+  int tw_key[3] = {-1, -1, -1};  // (nf1, nf2, sizeof(T)) of the twiddles held in S_TW
 };
 std::mutex g_ws_mu;
 std::vector<Workspace> g_ws;
@@ -737,6 +1014,7 @@ int ws_get(int dev, Slot s, size_t bytes, void **out) {
       return kErrAlloc;
     }
     w.n[s] = bytes;
+    if (s == S_TW) w.tw_key[0] = w.tw_key[1] = w.tw_key[2] = -1;
   }
   *out = w.p[s];
   return kOk;
@@ -850,13 +1128,6 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
     return kErrSize;
   }
   Tuning tn = tune ? *tune : Tuning();
-  int T1 = tn.T1 > 0 ? tn.T1 : (dbl ? 32 : 64);
-  int T2 = tn.T2 > 0 ? tn.T2 : 32;
-  g.T1 = std::min(T1, g.nf1);
-  g.T2 = std::min(T2, g.nf2);
-  g.ntx = (g.nf1 + g.T1 - 1) / g.T1;
-  g.nty = (g.nf2 + g.T2 - 1) / g.T2;
-  g.ntiles = g.ntx * g.nty;
   size_t plane = (size_t)g.nf1 * g.nf2 * (dbl ? 16 : 8);
   int Pb = tn.Pb;
   if (Pb <= 0) {
@@ -865,8 +1136,44 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
   }
   g.Pb = std::max(1, std::min(Pb, g.P));
   g.nbatch = (g.P + g.Pb - 1) / g.Pb;
-  if ((double)g.nbatch * g.ntiles > 1.5e9) {
-    set_error("too many bins (%d batches x %d tiles)", g.nbatch, g.ntiles);
+  g.Pbin = std::max(1, std::min(tn.Pbin > 0 ? tn.Pbin : 8, g.P));
+  g.nlayer = (g.P + g.Pbin - 1) / g.Pbin;
+  // pruned LDS FFT: power-of-two grids whose rows fit one workgroup's LDS
+  g.fft_lds = 0;
+  g.A2 = g.B2 = g.Wc1 = g.Wc2 = g.rpw = g.lds_row_bytes = g.Erow = 0;
+  auto pow2 = [](int n) { return n > 0 && (n & (n - 1)) == 0; };
+  const size_t cs = dbl ? 16 : 8;
+  const size_t lds_max = tn.lds_bytes > 0 ? (size_t)tn.lds_bytes : size_t(160) << 10;
+  if (tn.fft != 1 && pow2(g.nf1) && pow2(g.nf2) && g.nf1 >= 64 && g.nf1 <= 16384 && g.nf2 >= 256) {
+    int m = 0;
+    while ((1 << m) < g.nf2) ++m;
+    g.A2 = 1 << (m / 2);
+    g.B2 = g.nf2 / g.A2;
+    g.rpw = std::max(1, std::min(4096 / g.nf1, g.nf2));
+    g.Wc1 = std::min(64, std::max(1, 4096 / g.A2));
+    g.Wc2 = std::min(64, std::max(1, 4096 / g.B2));
+    size_t rb = (size_t)fftc::padded_size(g.rpw * g.nf1) * cs;
+    size_t c1 = (size_t)fftc::padded_size(g.A2 * g.Wc1) * cs, c2 = (size_t)fftc::padded_size(g.B2 * g.Wc2) * cs;
+    g.Erow = 16;
+    if (rb <= lds_max && c1 <= lds_max && c2 <= lds_max && g.rpw * g.nf1 / g.Erow <= 1024) {
+      g.fft_lds = 1;
+      g.lds_row_bytes = (int)rb;
+    }
+  }
+  // bins: full-width row strips for the LDS path (one workgroup owns grid rows),
+  // T1 x T2 tiles for the tile-owner spread
+  if (g.fft_lds) {
+    g.T1 = g.nf1;
+    g.T2 = std::min(tn.T2 > 0 ? tn.T2 : 8, g.nf2);
+  } else {
+    g.T1 = std::min(tn.T1 > 0 ? tn.T1 : (dbl ? 32 : 64), g.nf1);
+    g.T2 = std::min(tn.T2 > 0 ? tn.T2 : 32, g.nf2);
+  }
+  g.ntx = (g.nf1 + g.T1 - 1) / g.T1;
+  g.nty = (g.nf2 + g.T2 - 1) / g.T2;
+  g.ntiles = g.ntx * g.nty;
+  if ((double)g.nlayer * g.ntiles > 1.5e9) {
+    set_error("too many bins (%d layers x %d tiles)", g.nlayer, g.ntiles);
     return kErrSize;
   }
   double qx[kMaxQuadNodes], qg[kMaxQuadNodes];
@@ -877,6 +1184,8 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
 }
 
 namespace {
+
+int device_lds_bytes();
 
 // Shared prep: w range -> geometry; bins; sorted records.  Returns number of records.
 template <typename T>
@@ -915,8 +1224,10 @@ int prepare(const Problem &pb, hipStream_t st, const Tuning *tune, int dev, Geom
     }
     if (nvalid == 0) wmin = wmax = 0;
   }
+  Tuning tn = tune ? *tune : Tuning();
+  if (tn.lds_bytes <= 0) tn.lds_bytes = device_lds_bytes();
   CURIG_TRY(plan_geometry((int)nvalid, pb.nx, pb.ny, pb.fov, pb.freq, pb.eps, pb.sigma, pb.sign, wmin, wmax, pb.dbl,
-                          tune, &g, &ko));
+                          &tn, &g, &ko));
   // correction tables: fk1 [nf1/2+1], fk2 [nf2/2+1], quadrature nodes and weights [2 nquad]
   size_t n1 = g.nf1 / 2 + 1, n2 = g.nf2 / 2 + 1;
   std::vector<double> ht(n1 + n2 + 2 * kMaxQuadNodes);
@@ -927,7 +1238,7 @@ int prepare(const Problem &pb, hipStream_t st, const Tuning *tune, int dev, Geom
   tabs = (double *)vp;
   CURIG_HIP(hipMemcpyAsync(tabs, ht.data(), sizeof(double) * ht.size(), hipMemcpyHostToDevice, st));
 
-  const long nbins = (long)g.nbatch * g.ntiles;
+  const long nbins = (long)g.nlayer * g.ntiles;
   CURIG_TRY(ws_get(dev, S_HIST, sizeof(int) * (nbins + 1), &vp));
   int *hist = (int *)vp;
   CURIG_TRY(ws_get(dev, S_OFF, sizeof(int) * (nbins + 1), &vp));
@@ -1010,28 +1321,128 @@ int batch_fft(int dev, const Geom &g, int nb, int sign, void *grid, hipStream_t 
   return kOk;
 }
 
-// rows whose first plane lies in [lo, hi] (cyclic, length <= P) -> up to two record ranges
+// rows whose first plane lies in the cyclic interval [lo, lo+len) -> up to two
+// record ranges (hoff[l] = first record of bin layer l, layers of Pbin planes)
 void source_ranges(const Geom &g, const std::vector<int> &hoff, int lo, int len, int &a0, int &a1, int &b0,
                    int &b1) {
-  // batches touched by the cyclic plane interval [lo, lo+len)
   if (len > g.P) len = g.P;
-  int first = lo / g.Pb;
-  int last_plane = lo + len - 1;
+  const int first = lo / g.Pbin;
+  const int last_plane = lo + len - 1;
   a0 = a1 = b0 = b1 = 0;
   if (last_plane < g.P) {
-    int lastb = last_plane / g.Pb;
     a0 = hoff[first];
-    a1 = hoff[lastb + 1];
+    a1 = hoff[last_plane / g.Pbin + 1];
   } else {
     a0 = hoff[first];
-    a1 = hoff[g.nbatch];
-    int lastb = (last_plane - g.P) / g.Pb;
-    if (lastb >= first) lastb = first - 1;  // whole circle already covered
-    if (lastb >= 0) {
+    a1 = hoff[g.nlayer];
+    int lastl = (last_plane - g.P) / g.Pbin;
+    if (lastl >= first) lastl = first - 1;  // the whole circle is covered
+    if (lastl >= 0) {
       b0 = hoff[0];
-      b1 = hoff[lastb + 1];
+      b1 = hoff[lastl + 1];
     }
   }
+}
+
+int device_lds_bytes() {
+  static int cached = -1;
+  if (cached < 0) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      v = 65536;
+    cached = v;
+  }
+  return cached;
+}
+
+// Twiddle tables tw[t] = e^{+2 pi i t / n} for n = nf1 and n = nf2 (FP64 values,
+// rounded to T), uploaded when the sizes change.
+template <typename T>
+int twiddles(int dev, const Geom &g, hipStream_t st, const cpx<T> *&tw1, const cpx<T> *&tw2) {
+  void *vp;
+  const size_t bytes = sizeof(cpx<T>) * (size_t)(g.nf1 + g.nf2);
+  CURIG_TRY(ws_get(dev, S_TW, bytes, &vp));
+  cpx<T> *t = (cpx<T> *)vp;
+  int *key = g_ws[dev].tw_key;
+  if (key[0] != g.nf1 || key[1] != g.nf2 || key[2] != (int)sizeof(T)) {
+    std::vector<cpx<T>> h(g.nf1 + g.nf2);
+    for (int i = 0; i < g.nf1; ++i) h[i] = fftc::mk((T)std::cos(2 * M_PI * i / g.nf1), (T)std::sin(2 * M_PI * i / g.nf1));
+    for (int i = 0; i < g.nf2; ++i)
+      h[g.nf1 + i] = fftc::mk((T)std::cos(2 * M_PI * i / g.nf2), (T)std::sin(2 * M_PI * i / g.nf2));
+    CURIG_HIP(hipMemcpyAsync(t, h.data(), bytes, hipMemcpyHostToDevice, st));
+    CURIG_HIP(hipStreamSynchronize(st));
+    key[0] = g.nf1;
+    key[1] = g.nf2;
+    key[2] = (int)sizeof(T);
+  }
+  tw1 = t;
+  tw2 = t + g.nf1;
+  return kOk;
+}
+
+template <typename K>
+int allow_lds(K kernel, int bytes) {
+  if (bytes > 65536) CURIG_HIP(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  return kOk;
+}
+
+template <typename T, int E, int KWM>
+int launch_spread_row(const Rec<T> *recs, const int *off, cpx<T> *H, const cpx<T> *tw1, const Geom &g, int p0, int nb,
+                      hipStream_t st) {
+  CURIG_TRY(allow_lds(spread_row_fft_kernel<T, E, KWM>, g.lds_row_bytes));
+  spread_row_fft_kernel<T, E, KWM><<<dim3(g.nf2 / g.rpw, nb), g.rpw * g.nf1 / E, g.lds_row_bytes, st>>>(
+      recs, off, H, tw1, g, p0);
+  return kOk;
+}
+
+template <typename T, int E>
+int spread_row_dispatch(const Rec<T> *recs, const int *off, cpx<T> *H, const cpx<T> *tw1, const Geom &g, int p0,
+                        int nb, hipStream_t st) {
+  if (g.kw <= 8) return launch_spread_row<T, E, 8>(recs, off, H, tw1, g, p0, nb, st);
+  if (g.kw <= 16) return launch_spread_row<T, E, 16>(recs, off, H, tw1, g, p0, nb, st);
+  return launch_spread_row<T, E, kMaxKernelWidth>(recs, off, H, tw1, g, p0, nb, st);
+}
+
+// ms2dirty planes of one batch: spread + row FFT -> H, column passes -> compact Fc [nb][N2][N1]
+template <typename T>
+int lds_fft_forward(const Rec<T> *recs, const int *off, const Geom &g, int p0, int nb, cpx<T> *Fc, cpx<T> *H,
+                    const cpx<T> *tw1, const cpx<T> *tw2, hipStream_t st, Timer &tm, int &mark_fft) {
+  const int sign = +1;
+  CURIG_TRY((spread_row_dispatch<T, 16>(recs, off, H, tw1, g, p0, nb, st)));
+  mark_fft = tm.mark();
+  const int ncb1 = (g.N1 + g.Wc1 - 1) / g.Wc1, ncb2 = (g.N1 + g.Wc2 - 1) / g.Wc2;
+  const int l1 = fftc::padded_size(g.A2 * g.Wc1) * (int)sizeof(cpx<T>);
+  const int l2 = fftc::padded_size(g.B2 * g.Wc2) * (int)sizeof(cpx<T>);
+  CURIG_TRY(allow_lds(col_pass1_kernel<T, false>, l1));
+  col_pass1_kernel<T, false><<<dim3(g.B2 * ncb1, nb), g.A2 * g.Wc1 / 16, l1, st>>>(
+      H, H, tw2, g.nf2, g.N1, g.N2, g.A2, g.B2, g.Wc1, sign);
+  CURIG_TRY(allow_lds(col_pass2_kernel<T, true>, l2));
+  col_pass2_kernel<T, true><<<dim3(g.A2 * ncb2, nb), g.B2 * g.Wc2 / 16, l2, st>>>(
+      H, Fc, tw2, g.nf2, g.N1, g.N2, g.A2, g.B2, g.Wc2, sign);
+  CURIG_HIP(hipGetLastError());
+  return kOk;
+}
+
+// dirty2ms planes of one batch: compact Fc [nb][N2][N1] (in G's buffer) -> full planes G
+template <typename T>
+int lds_fft_backward(const Geom &g, int nb, cpx<T> *G, cpx<T> *H, cpx<T> *H2, const cpx<T> *tw1,
+                     const cpx<T> *tw2, hipStream_t st) {
+  const int sign = -1;
+  const int ncb1 = (g.N1 + g.Wc1 - 1) / g.Wc1, ncb2 = (g.N1 + g.Wc2 - 1) / g.Wc2;
+  const int l1 = fftc::padded_size(g.A2 * g.Wc1) * (int)sizeof(cpx<T>);
+  const int l2 = fftc::padded_size(g.B2 * g.Wc2) * (int)sizeof(cpx<T>);
+  CURIG_TRY(allow_lds(col_pass1_kernel<T, true>, l1));
+  col_pass1_kernel<T, true><<<dim3(g.B2 * ncb1, nb), g.A2 * g.Wc1 / 16, l1, st>>>(
+      G, H, tw2, g.nf2, g.N1, g.N2, g.A2, g.B2, g.Wc1, sign);
+  CURIG_TRY(allow_lds(col_pass2_kernel<T, false>, l2));
+  col_pass2_kernel<T, false><<<dim3(g.A2 * ncb2, nb), g.B2 * g.Wc2 / 16, l2, st>>>(
+      H, H2, tw2, g.nf2, g.N1, g.N2, g.A2, g.B2, g.Wc2, sign);
+  CURIG_TRY(allow_lds(row_ifft_expand_kernel<T, 16>, g.lds_row_bytes));
+  row_ifft_expand_kernel<T, 16><<<dim3(g.nf2 / g.rpw, nb), g.rpw * g.nf1 / 16, g.lds_row_bytes, st>>>(
+      H2, G, tw1, g.nf1, g.nf2, g.N1, g.rpw, sign);
+  CURIG_HIP(hipGetLastError());
+  return kOk;
 }
 
 template <typename T>
@@ -1053,22 +1464,37 @@ int ms2dirty_impl(const Problem &pb, hipStream_t st, const Tuning *tune, Stats *
   int e1 = tm.mark();
   void *vp;
   const size_t plane = (size_t)g.nf1 * g.nf2;
-  CURIG_TRY(ws_get(dev, S_GRID, sizeof(v2_t<T>) * plane * g.Pb, &vp));
-  v2_t<T> *grid = (v2_t<T> *)vp;
   const long npix = (long)g.N1 * g.N2;
+  CURIG_TRY(ws_get(dev, S_GRID, sizeof(v2_t<T>) * (g.fft_lds ? (size_t)npix : plane) * g.Pb, &vp));
+  v2_t<T> *grid = (v2_t<T> *)vp;
   CURIG_TRY(ws_get(dev, S_ACC, sizeof(double2) * npix, &vp));
   double2 *acc = (double2 *)vp;
   const double *fk1 = tabs, *fk2 = tabs + (g.nf1 / 2 + 1), *q = fk2 + (g.nf2 / 2 + 1);
+  cpx<T> *H = nullptr;
+  const cpx<T> *tw1 = nullptr, *tw2 = nullptr;
+  if (g.fft_lds) {
+    CURIG_TRY(ws_get(dev, S_H, sizeof(cpx<T>) * (size_t)g.nf2 * g.N1 * g.Pb, &vp));
+    H = (cpx<T> *)vp;
+    CURIG_TRY(twiddles<T>(dev, g, st, tw1, tw2));
+  }
   double t_spread = 0, t_fft = 0, t_wacc = 0;
   for (int b = 0; b < g.nbatch; ++b) {
     int p0 = b * g.Pb, nb = std::min(g.Pb, g.P - p0);
-    int a = tm.mark();
-    spread_dispatch<T>(recs, off, grid, g, p0, nb, st);
-    CURIG_HIP(hipGetLastError());
-    int bm = tm.mark();
-    CURIG_TRY(batch_fft<T>(dev, g, nb, +1, grid, st));
+    int a = tm.mark(), bm;
+    if (g.fft_lds) {
+      // the fused kernel's time counts as spread (t_grid); the column passes as FFT
+      CURIG_TRY(lds_fft_forward<T>(recs, off, g, p0, nb, (cpx<T> *)grid, H, tw1, tw2, st, tm, bm));
+    } else {
+      spread_dispatch<T>(recs, off, grid, g, p0, nb, st);
+      CURIG_HIP(hipGetLastError());
+      bm = tm.mark();
+      CURIG_TRY(batch_fft<T>(dev, g, nb, +1, grid, st));
+    }
     int c = tm.mark();
-    wacc_kernel<T><<<nblocks(npix), 256, 0, st>>>(grid, acc, g, p0, nb, b == 0 ? 1 : 0);
+    if (g.fft_lds)
+      wacc_compact_kernel<T><<<nblocks(npix), 256, 0, st>>>((const cpx<T> *)grid, acc, g, p0, nb, b == 0 ? 1 : 0);
+    else
+      wacc_kernel<T><<<nblocks(npix), 256, 0, st>>>(grid, acc, g, p0, nb, b == 0 ? 1 : 0);
     CURIG_HIP(hipGetLastError());
     int d = tm.mark();
     if (tm.on) {
@@ -1089,6 +1515,7 @@ int ms2dirty_impl(const Problem &pb, hipStream_t st, const Tuning *tune, Stats *
     CURIG_HIP(hipStreamSynchronize(st));
     sts->kw = g.kw; sts->nf1 = g.nf1; sts->nf2 = g.nf2; sts->P = g.P; sts->Pb = g.Pb; sts->nbatch = g.nbatch;
     sts->T1 = g.T1; sts->T2 = g.T2; sts->nvis = nrec; sts->beta = g.beta; sts->gamma = g.gamma;
+    sts->fft_lds = g.fft_lds;
     sts->C = g.C; sts->D = g.D;
     sts->t_prep = tm.ms(e0, e1);
     sts->t_grid = t_spread;
@@ -1124,9 +1551,15 @@ int dirty2ms_impl(const Problem &pb, hipStream_t st, const Tuning *tune, Stats *
     else
       zero_rows_kernel<float><<<nblocks(pb.nrow), 256, 0, st>>>(pb.vis, pb.vis_stride, pb.nrow);
   }
-  std::vector<int> hoff(g.nbatch + 1);
-  for (int b = 0; b <= g.nbatch; ++b)
-    CURIG_HIP(hipMemcpyAsync(&hoff[b], off + (long)b * g.ntiles, sizeof(int), hipMemcpyDeviceToHost, st));
+  std::vector<int> hoff(g.nlayer + 1);
+  {
+    // first record of every bin layer, gathered on the device and copied once
+    void *hp;
+    CURIG_TRY(ws_get(dev, S_CNT, sizeof(int) * (g.nlayer + 1), &hp));
+    gather_layers_kernel<<<nblocks(g.nlayer + 1), 256, 0, st>>>(off, g.ntiles, g.nlayer, (int *)hp);
+    CURIG_HIP(hipGetLastError());
+    CURIG_HIP(hipMemcpyAsync(hoff.data(), hp, sizeof(int) * (g.nlayer + 1), hipMemcpyDeviceToHost, st));
+  }
   void *vp;
   const size_t plane = (size_t)g.nf1 * g.nf2;
   CURIG_TRY(ws_get(dev, S_GRID, sizeof(v2_t<T>) * plane * g.Pb, &vp));
@@ -1145,14 +1578,29 @@ int dirty2ms_impl(const Problem &pb, hipStream_t st, const Tuning *tune, Stats *
   CURIG_HIP(hipGetLastError());
   CURIG_HIP(hipStreamSynchronize(st));  // hoff
   int e1 = tm.mark();
+  cpx<T> *H = nullptr, *H2 = nullptr;
+  const cpx<T> *tw1 = nullptr, *tw2 = nullptr;
+  if (g.fft_lds) {
+    CURIG_TRY(ws_get(dev, S_H, sizeof(cpx<T>) * (size_t)g.nf2 * g.N1 * g.Pb, &vp));
+    H = (cpx<T> *)vp;
+    CURIG_TRY(ws_get(dev, S_H2, sizeof(cpx<T>) * (size_t)g.nf2 * g.N1 * g.Pb, &vp));
+    H2 = (cpx<T> *)vp;
+    CURIG_TRY(twiddles<T>(dev, g, st, tw1, tw2));
+  }
   double t_interp = 0, t_fft = 0, t_fill = 0;
   for (int b = 0; b < g.nbatch; ++b) {
     int p0 = b * g.Pb, nb = std::min(g.Pb, g.P - p0);
     int a = tm.mark();
-    wfill_kernel<T><<<nblocks((long)plane), 256, 0, st>>>(gimg, grid, g, p0, nb);
+    if (g.fft_lds)
+      wfill_compact_kernel<T><<<nblocks(npix), 256, 0, st>>>(gimg, (cpx<T> *)grid, g, p0, nb);
+    else
+      wfill_kernel<T><<<nblocks((long)plane), 256, 0, st>>>(gimg, grid, g, p0, nb);
     CURIG_HIP(hipGetLastError());
     int bm = tm.mark();
-    CURIG_TRY(batch_fft<T>(dev, g, nb, -1, grid, st));
+    if (g.fft_lds)
+      CURIG_TRY(lds_fft_backward<T>(g, nb, (cpx<T> *)grid, H, H2, tw1, tw2, st));
+    else
+      CURIG_TRY(batch_fft<T>(dev, g, nb, -1, grid, st));
     int c = tm.mark();
     int ra0, ra1, rb0, rb1;
     source_ranges(g, hoff, imod_host(p0 - g.kw + 1, g.P), nb + g.kw - 1, ra0, ra1, rb0, rb1);
@@ -1181,6 +1629,7 @@ int dirty2ms_impl(const Problem &pb, hipStream_t st, const Tuning *tune, Stats *
     CURIG_HIP(hipStreamSynchronize(st));
     sts->kw = g.kw; sts->nf1 = g.nf1; sts->nf2 = g.nf2; sts->P = g.P; sts->Pb = g.Pb; sts->nbatch = g.nbatch;
     sts->T1 = g.T1; sts->T2 = g.T2; sts->nvis = nrec; sts->beta = g.beta; sts->gamma = g.gamma;
+    sts->fft_lds = g.fft_lds;
     sts->C = g.C; sts->D = g.D;
     sts->t_prep = tm.ms(e0, e1);
     sts->t_grid = t_interp;

@@ -61,9 +61,10 @@ int curig_explicit_ms2dirty_dev(int nrow, int nxdirty, int nydirty, double fov, 
 const char *curig_last_error(void);
 
 /* Host-only plan computation (no device access): for nrow_valid unflagged rows
- * whose w (metres) spans [wmin, wmax].  ints[8] = kw, nf1, nf2, nf3 (w planes),
- * planes per batch, batches, tile width, tile height; reals[8] = beta, c, gamma,
- * h, i_center, o_center, pixelsize_x, pixelsize_y. */
+ * whose w (metres) spans [wmin, wmax].  ints[9] = kw, nf1, nf2, nf3 (w planes),
+ * planes per batch, batches, tile width, tile height, FFT path (1: pruned LDS
+ * FFT, 0: rocFFT); reals[8] = beta, c, gamma, h, i_center, o_center,
+ * pixelsize_x, pixelsize_y. */
 int curig_plan_info(int nrow_valid, int nxdirty, int nydirty, double fov, double freq, double epsilon,
                     double sigma, int sign, double wmin, double wmax, int precision, int *ints, double *reals);
 /* Host-only: correction factors phihat(k) (-1)^k, k = 0..nf/2, of the kernel
@@ -75,10 +76,13 @@ int curig_gauss_legendre(int n, double *x, double *w);
 /* Tile width / height and w planes per batch used by later calls (0 = default).
  * The autotuner (curig_amd.autotune) sets these. */
 int curig_set_tuning(int tile_x, int tile_y, int planes_per_batch);
+/* FFT path of later calls: 0 = pruned LDS FFT where the grid is a power of two
+ * and a row fits one workgroup's LDS (else rocFFT), 1 = always rocFFT. */
+int curig_set_fft_path(int path);
 /* 1: time the stages of later calls with hipEvents (adds synchronisation). */
 int curig_enable_stats(int on);
-/* Stats of the last call on this process: ints[8] = kw, nf1, nf2, nf3, planes
- * per batch, batches, tile width, tile height; reals[8] = rows gridded, t_prep,
+/* Stats of the last call on this process: ints[9] = kw, nf1, nf2, nf3, planes
+ * per batch, batches, tile width, tile height, FFT path; reals[8] = rows gridded, t_prep,
  * t_grid (spread / interp), t_fft, t_plane (w accumulate / w fill), t_final,
  * t_total (ms), timed flag. */
 int curig_get_stats(int *ints, double *reals);

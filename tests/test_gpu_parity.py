@@ -355,3 +355,50 @@ def test_small_plane_batches_match_restatement(planes_per_batch):
         assert O.l2error(R.ms2dirty(uvw, vis, wgt, nx, ny, fov, f0, eps), d32) < 3e-6
     finally:
         lib.curig_set_tuning(0, 0, 0)
+
+
+@pytest.mark.parametrize("nx,ny", [(128, 128), (256, 128), (128, 512), (512, 512)])
+def test_lds_fft_path_matches_restatement(nx, ny):
+    """Power-of-two grids run the pruned LDS FFT (row pass + four-step column
+    passes); FP64 against the restatement, FP32 against the direct sum."""
+    from curig_amd import _lib as L
+
+    say(f"[lds fft] {nx}x{ny}")
+    uvw, vis, wgt, f0, rng = rand_case(nx * 7 + ny, 600, nx, 1.4, wscale=6.0)
+    fov, eps = 3.0, 1e-7
+    info = L.plan_info(600, nx, ny, fov, f0, eps, 2.0, -1, uvw[:, 2].min(), uvw[:, 2].max(), 0)
+    assert info["fft_lds"] == 1, info
+    img = rng.standard_normal((nx, ny)) + 1j * rng.standard_normal((nx, ny))
+    d = gpu_ms2dirty(uvw, vis, wgt, nx, ny, fov, f0, eps)
+    assert L.stats()["fft_lds"] in (0, 1)
+    assert O.l2error(R.ms2dirty(uvw, vis, wgt, nx, ny, fov, f0, eps), d) < 1e-10
+    v = gpu_dirty2ms(uvw, img, wgt, nx, ny, fov, f0, eps)
+    assert O.l2error(R.dirty2ms(uvw, img, wgt, nx, ny, fov, f0, eps), v) < 1e-10
+    np.testing.assert_allclose(np.vdot(vis, v), np.vdot(d, img), rtol=1e-12)
+    d32 = gpu_f32(True, uvw, vis, wgt, nx, ny, fov, f0, 1e-6)
+    assert O.l2error(O.dft_ms2dirty(uvw, vis, wgt, nx, ny, fov, f0, -1), d32) < 3e-6 + 3e-6
+    v32 = gpu_f32(False, uvw, img, wgt, nx, ny, fov, f0, 1e-6)
+    assert O.l2error(O.dft_dirty2ms(uvw, img, wgt, nx, ny, fov, f0, -1), v32) < 6e-6
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_lds_fft_path_equals_rocfft_path(precision):
+    """The two FFT paths on one 2048^2 problem (many rows, deep w range)."""
+    lib = _lib()
+    nx = ny = 2048
+    say(f"[lds vs rocfft] {precision} {nx}x{ny}")
+    uvw, vis, wgt, f0, rng = rand_case(99, 200000, nx, 1.2, wscale=40.0)
+    img = rng.standard_normal((nx, ny)) + 1j * rng.standard_normal((nx, ny))
+    fn_d = gpu_ms2dirty if precision == "fp64" else (lambda *a: gpu_f32(True, *a))
+    fn_v = gpu_dirty2ms if precision == "fp64" else (lambda *a: gpu_f32(False, *a))
+    d_lds = fn_d(uvw, vis, wgt, nx, ny, 2.0, f0, 1e-6)
+    v_lds = fn_v(uvw, img, wgt, nx, ny, 2.0, f0, 1e-6)
+    assert lib.curig_set_fft_path(1) == 0
+    try:
+        d_roc = fn_d(uvw, vis, wgt, nx, ny, 2.0, f0, 1e-6)
+        v_roc = fn_v(uvw, img, wgt, nx, ny, 2.0, f0, 1e-6)
+    finally:
+        lib.curig_set_fft_path(0)
+    tol = 1e-11 if precision == "fp64" else 2e-6
+    assert O.l2error(d_roc, d_lds) < tol
+    assert O.l2error(v_roc, v_lds) < tol

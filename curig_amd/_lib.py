@@ -21,7 +21,7 @@ EXPORTS = [
     "ms2dirty_f32", "dirty2ms_f32", "curig_ms2dirty_dev", "curig_dirty2ms_dev",
     "curig_ms2dirty_multi_dev", "curig_dirty2ms_multi_dev", "curig_explicit_ms2dirty_dev",
     "curig_last_error", "curig_plan_info", "curig_kernel_fseries", "curig_gauss_legendre",
-    "curig_set_tuning", "curig_set_fft_path", "curig_enable_stats", "curig_get_stats", "curig_release",
+    "curig_set_tuning", "curig_set_fft_path", "curig_set_graph_mode", "curig_enable_stats", "curig_get_stats", "curig_release",
 ]
 # C++-linkage plan API (include/cugridder.h), exported with mangled names
 CXX_EXPORTS = ["gridder_setting", "gridder_execution", "gridder_destroy"]
@@ -91,6 +91,8 @@ def load():
     lib.curig_set_tuning.restype = c_int
     lib.curig_set_fft_path.argtypes = [c_int]
     lib.curig_set_fft_path.restype = c_int
+    lib.curig_set_graph_mode.argtypes = [c_int]
+    lib.curig_set_graph_mode.restype = c_int
     lib.curig_enable_stats.argtypes = [c_int]
     lib.curig_enable_stats.restype = c_int
     lib.curig_get_stats.argtypes = [np.ctypeslib.ndpointer(np.int32, flags="C"), _f64]

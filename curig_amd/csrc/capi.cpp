@@ -216,6 +216,8 @@ extern "C" int curig_dirty2ms_dev(int nrow, int nxdirty, int nydirty, double fov
   return dirty2ms_device(pb, (hipStream_t)stream, &g_tuning, stats_ptr());
 }
 
+static int g_graph_mode = 1;
+
 static int multi(bool to_image, int nrow, int nchan, int nx, int ny, double fov, const double *freq,
                  const double *d_uvw, const void *d_vis, const void *d_wgt, const uint8_t *d_flag, const void *d_dirty,
                  int per_channel, double eps, double sigma, int sign, int precision, void *stream) {
@@ -226,7 +228,7 @@ static int multi(bool to_image, int nrow, int nchan, int nx, int ny, double fov,
     return kErrArg;
   }
   const size_t cs = precision == 0 ? 16 : 8, rs = precision == 0 ? 8 : 4;
-  Stats total;
+  std::vector<Problem> pbs(nchan);
   for (int c = 0; c < nchan; ++c) {
     Problem pb = dev_problem(nrow, nx, ny, fov, freq[c], d_uvw, (const char *)d_vis + cs * c,
                              d_wgt ? (const char *)d_wgt + rs * c : nullptr, d_dirty, eps, sigma, sign, precision);
@@ -236,34 +238,9 @@ static int multi(bool to_image, int nrow, int nchan, int nx, int ny, double fov,
     pb.flag_stride = nchan;
     if (per_channel) pb.dirty = (char *)const_cast<void *>(d_dirty) + cs * (size_t)nx * ny * c;
     pb.accumulate = to_image && !per_channel && c > 0;
-    int ier = to_image ? ms2dirty_device(pb, (hipStream_t)stream, &g_tuning, stats_ptr())
-                       : dirty2ms_device(pb, (hipStream_t)stream, &g_tuning, stats_ptr());
-    if (ier != kOk) return ier;
-    if (g_stats_on) {
-      total.nvis += g_last_stats.nvis;
-      total.t_prep += g_last_stats.t_prep;
-      total.t_grid += g_last_stats.t_grid;
-      total.t_fft += g_last_stats.t_fft;
-      total.t_plane += g_last_stats.t_plane;
-      total.t_final += g_last_stats.t_final;
-      total.t_total += g_last_stats.t_total;
-      total.P += g_last_stats.P;
-    }
+    pbs[c] = pb;
   }
-  if (g_stats_on && nchan > 1) {
-    Stats last = g_last_stats;
-    g_last_stats = total;
-    g_last_stats.kw = last.kw;
-    g_last_stats.nf1 = last.nf1;
-    g_last_stats.nf2 = last.nf2;
-    g_last_stats.Pb = last.Pb;
-    g_last_stats.nbatch = last.nbatch;
-    g_last_stats.T1 = last.T1;
-    g_last_stats.T2 = last.T2;
-    g_last_stats.fft_lds = last.fft_lds;
-    g_last_stats.timed = 1;
-  }
-  return kOk;
+  return multi_device(to_image, pbs.data(), nchan, (hipStream_t)stream, &g_tuning, g_graph_mode, stats_ptr());
 }
 
 extern "C" int curig_ms2dirty_multi_dev(int nrow, int nchan, int nxdirty, int nydirty, double fov, const double *freq,
@@ -358,6 +335,11 @@ extern "C" int curig_set_fft_path(int path) {
     return kErrArg;
   }
   g_tuning.fft = path;
+  return kOk;
+}
+
+extern "C" int curig_set_graph_mode(int on) {
+  g_graph_mode = on != 0;
   return kOk;
 }
 

@@ -38,6 +38,13 @@ struct Stats {
 int ms2dirty_device(const Problem &pb, hipStream_t stream, const Tuning *tune, Stats *st);
 int dirty2ms_device(const Problem &pb, hipStream_t stream, const Tuning *tune, Stats *st);
 
+// Several problems (channels) sharing image size, epsilon and sigma, queued in
+// order on one stream.  graph_mode != 0: the whole sequence is captured once into
+// a hipGraph and replayed while the same call repeats (LDS FFT path only; other
+// grids run the plain per-channel loop).  Stats: totals over the problems.
+int multi_device(bool to_image, const Problem *pbs, int n, hipStream_t stream, const Tuning *tune, int graph_mode,
+                 Stats *st);
+
 // FP64 direct sum on the GPU (the reference's explicit_gridder, precomp.cu:242-275),
 // at npix chosen pixels (pix = i*ny + k) or, with pix == nullptr, at every pixel.
 int explicit_ms2dirty_device(const Problem &pb, long npix, const int64_t *pix, void *out,

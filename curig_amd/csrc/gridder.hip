@@ -60,13 +60,17 @@ template <> struct __align__(8) Rec<double> {
 // --------------------------------------------------------------------------
 // device helpers
 // --------------------------------------------------------------------------
+// Normalised ES kernel exp(beta (sqrt(1 - c d^2) - 1)), 0 for |d| >= kw/2.  The
+// exponent is formed as -beta c d^2 / (sqrt(1 - c d^2) + 1): no cancellation, so
+// the FP32 value keeps ~1e-7 relative accuracy near the peak (beta ~ 10-30 would
+// amplify the rounding of sqrt(.) - 1 to ~1e-6).
 __device__ __forceinline__ float es_eval(float d, float beta, float c, float hw) {
-  float s = 1.0f - c * d * d;
-  return (fabsf(d) < hw && s > 0.0f) ? expf(beta * (sqrtf(s) - 1.0f)) : 0.0f;
+  const float t = c * d * d;
+  return (fabsf(d) < hw && t < 1.0f) ? expf(-beta * t / (sqrtf(1.0f - t) + 1.0f)) : 0.0f;
 }
 __device__ __forceinline__ double es_eval(double d, double beta, double c, double hw) {
-  double s = 1.0 - c * d * d;
-  return (fabs(d) < hw && s > 0.0) ? exp(beta * (sqrt(s) - 1.0)) : 0.0;
+  const double t = c * d * d;
+  return (fabs(d) < hw && t < 1.0) ? exp(-beta * t / (sqrt(1.0 - t) + 1.0)) : 0.0;
 }
 
 __device__ __forceinline__ int imod(int a, int n) {
@@ -200,11 +204,6 @@ __global__ void __launch_bounds__(256) w_minmax_kernel(const double *__restrict_
     part[2 * blockIdx.x + 1] = smax[0];
     cnt[blockIdx.x] = scnt[0];
   }
-}
-
-__global__ void gather_layers_kernel(const int *__restrict__ off, int ntiles, int nlayer, int *__restrict__ out) {
-  for (long l = blockIdx.x * (long)blockDim.x + threadIdx.x; l <= nlayer; l += (long)gridDim.x * blockDim.x)
-    out[l] = off[l * ntiles];
 }
 
 __global__ void fill_int_kernel(int *__restrict__ p, long n, int v) {
@@ -586,14 +585,38 @@ __global__ void __launch_bounds__(256) wfill_kernel(const double2 *__restrict__ 
   }
 }
 
+// Records whose first plane lies in the cyclic interval [lo, lo + len) (len <= P):
+// up to two ranges of bin layers, read from the bin offsets on the device.
+__device__ __forceinline__ void layer_ranges(const int *__restrict__ off, const Geom &g, int lo, int len, int &a0,
+                                             int &a1, int &b0, int &b1) {
+  if (len > g.P) len = g.P;
+  const int first = lo / g.Pbin;
+  const int last_plane = lo + len - 1;
+  a0 = off[(long)first * g.ntiles];
+  b0 = b1 = 0;
+  if (last_plane < g.P) {
+    a1 = off[(long)(last_plane / g.Pbin + 1) * g.ntiles];
+  } else {
+    a1 = off[(long)g.nlayer * g.ntiles];
+    int lastl = (last_plane - g.P) / g.Pbin;
+    if (lastl >= first) lastl = first - 1;  // the whole circle is covered
+    if (lastl >= 0) {
+      b0 = off[0];
+      b1 = off[(long)(lastl + 1) * g.ntiles];
+    }
+  }
+}
+
 // interpolation (dirty2ms): per record, sum over the kernel support inside the
 // batch's planes; FP32 (or FP64) multiply-accumulate per plane, FP64 across planes.
 // src/fourier/interp.cu:143-228.
 template <typename T, int KWM>
 __global__ void __launch_bounds__(256) interp_kernel(const Rec<T> *__restrict__ recs,
+                                                      const int *__restrict__ off,
                                                       const v2_t<T> *__restrict__ grid,
-                                                      double2 *__restrict__ vacc, Geom g, int p0, int nb,
-                                                      int ra0, int ra1, int rb0, int rb1) {
+                                                      double2 *__restrict__ vacc, Geom g, int p0, int nb) {
+  int ra0, ra1, rb0, rb1;
+  layer_ranges(off, g, imod(p0 - g.kw + 1, g.P), nb + g.kw - 1, ra0, ra1, rb0, rb1);
   const long n0 = ra1 - ra0, ntot = n0 + (rb1 - rb0);
   const size_t plane = (size_t)g.nf1 * g.nf2;
   const int kw = g.kw;
@@ -624,27 +647,29 @@ __global__ void __launch_bounds__(256) interp_kernel(const Rec<T> *__restrict__ 
       T pr = 0, pi = 0;
 #pragma unroll
       for (int jj = 0; jj < KWM; ++jj) {
-        if (jj >= kw) break;
-        int cy = rc.iy + jj;
-        if (cy >= g.nf2) cy -= g.nf2;
-        const v2_t<T> *rowp = pb + (size_t)cy * g.nf1;
-        T rr = 0, ri = 0;
+        if (jj < kw) {
+          int cy = rc.iy + jj;
+          if (cy >= g.nf2) cy -= g.nf2;
+          const v2_t<T> *rowp = pb + (size_t)cy * g.nf1;
+          T rr = 0, ri = 0;
 #pragma unroll
-        for (int i = 0; i < KWM; ++i) {
-          if (i >= kw) break;
-          int cx = rc.ix + i;
-          if (cx >= g.nf1) cx -= g.nf1;
-          v2_t<T> f = rowp[cx];
-          rr += f.x * kx[i];
-          ri += f.y * kx[i];
+          for (int i = 0; i < KWM; ++i) {
+            if (i < kw) {
+              int cx = rc.ix + i;
+              if (cx >= g.nf1) cx -= g.nf1;
+              const v2_t<T> f = rowp[cx];
+              rr += f.x * kx[i];
+              ri += f.y * kx[i];
+            }
+          }
+          pr += rr * ky[jj];
+          pi += ri * ky[jj];
         }
-        pr += rr * ky[jj];
-        pi += ri * ky[jj];
       }
       sr += (double)(pr * kz);
       si += (double)(pi * kz);
     }
-    double2 o = vacc[r];
+    const double2 o = vacc[r];
     vacc[r] = make_double2(o.x + sr, o.y + si);
   }
 }
@@ -1185,33 +1210,67 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
 
 namespace {
 
-int device_lds_bytes();
+int device_lds_bytes() {
+  static int cached = -1;
+  if (cached < 0) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      v = 65536;
+    cached = v;
+  }
+  return cached;
+}
 
-// Shared prep: w range -> geometry; bins; sorted records.  Returns number of records.
-template <typename T>
-int prepare(const Problem &pb, hipStream_t st, const Tuning *tune, int dev, Geom &g, KernelOpts &ko,
-            Rec<T> *&recs, int *&perm, int *&off, int &nrec, double *&tabs, bool with_value) {
-  const int nrow = pb.nrow;
-  const int nbm = std::min(nblocks(nrow), 1024);
-  void *vp;
-  CURIG_TRY(ws_get(dev, S_PART, sizeof(double) * 2 * nbm, &vp));
-  double *part = (double *)vp;
-  CURIG_TRY(ws_get(dev, S_CNT, sizeof(int) * nbm, &vp));
-  int *cnt = (int *)vp;
-  double wmin = 0, wmax = 0;
+struct ChanPlan {
+  Geom g;
+  KernelOpts ko;
   long nvalid = 0;
-  if (nrow > 0) {
-    w_minmax_kernel<<<nbm, 256, 0, st>>>(pb.uvw, nrow, pb.flag, pb.flag_stride, part, cnt);
-    CURIG_HIP(hipGetLastError());
-    std::vector<double> hp(2 * nbm);
-    std::vector<int> hc(nbm);
-    CURIG_HIP(hipMemcpyAsync(hp.data(), part, sizeof(double) * 2 * nbm, hipMemcpyDeviceToHost, st));
-    CURIG_HIP(hipMemcpyAsync(hc.data(), cnt, sizeof(int) * nbm, hipMemcpyDeviceToHost, st));
-    CURIG_HIP(hipStreamSynchronize(st));
-    wmin = INFINITY;
-    wmax = -INFINITY;
+};
+
+// Device pointers of one call's working set (slots of the per-device workspace).
+struct Bufs {
+  int dev = 0;
+  double *tabs = nullptr;  // fk1 [nf1/2+1] | fk2 [nf2/2+1] | quadrature nodes | weights
+  int *hist = nullptr, *off = nullptr, *bsum = nullptr, *perm = nullptr;
+  int2 *binrank = nullptr;
+  void *recs = nullptr, *grid = nullptr, *H = nullptr, *H2 = nullptr;
+  double2 *acc = nullptr, *vacc = nullptr;
+  const void *tw1 = nullptr, *tw2 = nullptr;
+};
+
+// Phase 1 (one wait): w range and unflagged count of every problem, then their geometries.
+int plan_all(const Problem *pbs, int n, hipStream_t st, const Tuning *tune, int dev, std::vector<ChanPlan> &cps) {
+  std::vector<int> nbm(n), base(n + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    nbm[i] = pbs[i].nrow > 0 ? std::min(nblocks(pbs[i].nrow), 1024) : 0;
+    base[i + 1] = base[i] + nbm[i];
+  }
+  void *vp;
+  CURIG_TRY(ws_get(dev, S_PART, sizeof(double) * 2 * std::max(base[n], 1), &vp));
+  double *part = (double *)vp;
+  CURIG_TRY(ws_get(dev, S_CNT, sizeof(int) * std::max(base[n], 1), &vp));
+  int *cnt = (int *)vp;
+  for (int i = 0; i < n; ++i)
+    if (nbm[i] > 0)
+      w_minmax_kernel<<<nbm[i], 256, 0, st>>>(pbs[i].uvw, pbs[i].nrow, pbs[i].flag, pbs[i].flag_stride,
+                                               part + 2 * base[i], cnt + base[i]);
+  CURIG_HIP(hipGetLastError());
+  std::vector<double> hp(2 * std::max(base[n], 1));
+  std::vector<int> hc(std::max(base[n], 1));
+  if (base[n] > 0) {
+    CURIG_HIP(hipMemcpyAsync(hp.data(), part, sizeof(double) * 2 * base[n], hipMemcpyDeviceToHost, st));
+    CURIG_HIP(hipMemcpyAsync(hc.data(), cnt, sizeof(int) * base[n], hipMemcpyDeviceToHost, st));
+  }
+  CURIG_HIP(hipStreamSynchronize(st));
+  Tuning tn = tune ? *tune : Tuning();
+  if (tn.lds_bytes <= 0) tn.lds_bytes = device_lds_bytes();
+  cps.assign(n, ChanPlan());
+  for (int i = 0; i < n; ++i) {
+    double wmin = INFINITY, wmax = -INFINITY;
+    long nvalid = 0;
     bool bad = false;
-    for (int b = 0; b < nbm; ++b) {
+    for (int b = base[i]; b < base[i + 1]; ++b) {
       nvalid += hc[b];
       if (hc[b] == 0) continue;
       if (std::isnan(hp[2 * b]) || std::isnan(hp[2 * b + 1])) bad = true;
@@ -1223,137 +1282,12 @@ int prepare(const Problem &pb, hipStream_t st, const Tuning *tune, int dev, Geom
       return kErrNonFinite;
     }
     if (nvalid == 0) wmin = wmax = 0;
-  }
-  Tuning tn = tune ? *tune : Tuning();
-  if (tn.lds_bytes <= 0) tn.lds_bytes = device_lds_bytes();
-  CURIG_TRY(plan_geometry((int)nvalid, pb.nx, pb.ny, pb.fov, pb.freq, pb.eps, pb.sigma, pb.sign, wmin, wmax, pb.dbl,
-                          &tn, &g, &ko));
-  // correction tables: fk1 [nf1/2+1], fk2 [nf2/2+1], quadrature nodes and weights [2 nquad]
-  size_t n1 = g.nf1 / 2 + 1, n2 = g.nf2 / 2 + 1;
-  std::vector<double> ht(n1 + n2 + 2 * kMaxQuadNodes);
-  fseries_grid(ko, g.nf1, ht.data());
-  fseries_grid(ko, g.nf2, ht.data() + n1);
-  kernel_quadrature(ko, ht.data() + n1 + n2, ht.data() + n1 + n2 + g.nquad);
-  CURIG_TRY(ws_get(dev, S_TAB, sizeof(double) * ht.size(), &vp));
-  tabs = (double *)vp;
-  CURIG_HIP(hipMemcpyAsync(tabs, ht.data(), sizeof(double) * ht.size(), hipMemcpyHostToDevice, st));
-
-  const long nbins = (long)g.nlayer * g.ntiles;
-  CURIG_TRY(ws_get(dev, S_HIST, sizeof(int) * (nbins + 1), &vp));
-  int *hist = (int *)vp;
-  CURIG_TRY(ws_get(dev, S_OFF, sizeof(int) * (nbins + 1), &vp));
-  off = (int *)vp;
-  const long nscan = (nbins + kScanTile - 1) / kScanTile;
-  CURIG_TRY(ws_get(dev, S_BSUM, sizeof(int) * (nscan + 1), &vp));
-  int *bsum = (int *)vp;
-  CURIG_TRY(ws_get(dev, S_BINRANK, sizeof(int2) * std::max(nrow, 1), &vp));
-  int2 *binrank = (int2 *)vp;
-  CURIG_TRY(ws_get(dev, S_RECS, sizeof(Rec<T>) * std::max(nrow, 1), &vp));
-  recs = (Rec<T> *)vp;
-  CURIG_TRY(ws_get(dev, S_PERM, sizeof(int) * std::max(nrow, 1), &vp));
-  perm = (int *)vp;
-
-  fill_int_kernel<<<nblocks(nbins), 256, 0, st>>>(hist, nbins, 0);
-  if (nrow > 0)
-    bin_count_kernel<<<nblocks(nrow), 256, 0, st>>>(pb.uvw, nrow, pb.flag, pb.flag_stride, g, hist, binrank);
-  scan_reduce_kernel<<<(int)nscan, kScanBlock, 0, st>>>(hist, nbins, bsum);
-  scan_sums_kernel<<<1, 1024, 0, st>>>(bsum, (int)nscan);
-  scan_apply_kernel<<<(int)nscan, kScanBlock, 0, st>>>(hist, nbins, bsum, off, (int)nvalid);
-  if (nrow > 0)
-    bin_scatter_kernel<T><<<nblocks(nrow), 256, 0, st>>>(pb.uvw, nrow, pb.vis, pb.vis_stride, pb.wgt,
-                                                          pb.wgt_stride, g, binrank, off, recs, perm,
-                                                          with_value ? 1 : 0);
-  CURIG_HIP(hipGetLastError());
-  nrec = (int)nvalid;
-  return kOk;
-}
-
-template <typename T, int KWM>
-void launch_spread(const Rec<T> *recs, const int *off, v2_t<T> *grid, const Geom &g, int p0, int nb,
-                   hipStream_t st) {
-  dim3 grd(g.ntiles, nb);
-  size_t lds = sizeof(T) * 2 * g.T1 * g.T2;
-  spread_tile_kernel<T, KWM><<<grd, 256, lds, st>>>(recs, off, grid, g, p0);
-}
-
-template <typename T>
-void spread_dispatch(const Rec<T> *recs, const int *off, v2_t<T> *grid, const Geom &g, int p0, int nb,
-                     hipStream_t st) {
-  if (g.kw <= 8)
-    launch_spread<T, 8>(recs, off, grid, g, p0, nb, st);
-  else if (g.kw <= 16)
-    launch_spread<T, 16>(recs, off, grid, g, p0, nb, st);
-  else
-    launch_spread<T, kMaxKernelWidth>(recs, off, grid, g, p0, nb, st);
-}
-
-template <typename T>
-void interp_dispatch(const Rec<T> *recs, const v2_t<T> *grid, double2 *vacc, const Geom &g, int p0, int nb,
-                     int ra0, int ra1, int rb0, int rb1, hipStream_t st) {
-  long n = (long)(ra1 - ra0) + (rb1 - rb0);
-  if (n <= 0) return;
-  int nbk = nblocks(n);
-  if (g.kw <= 8)
-    interp_kernel<T, 8><<<nbk, 256, 0, st>>>(recs, grid, vacc, g, p0, nb, ra0, ra1, rb0, rb1);
-  else if (g.kw <= 16)
-    interp_kernel<T, 16><<<nbk, 256, 0, st>>>(recs, grid, vacc, g, p0, nb, ra0, ra1, rb0, rb1);
-  else
-    interp_kernel<T, kMaxKernelWidth><<<nbk, 256, 0, st>>>(recs, grid, vacc, g, p0, nb, ra0, ra1, rb0, rb1);
-}
-
-// FFT for a batch of nb planes, at most kMaxFftElems elements per rocFFT call
-// (the reference caps cuFFT batches the same way, MAX_CUFFT_ELEM, cugridder.cu:230-246;
-// a single call over 2^32 elements returns wrong planes).
-constexpr size_t kMaxFftElems = size_t(1) << 28;
-
-template <typename T>
-int batch_fft(int dev, const Geom &g, int nb, int sign, void *grid, hipStream_t st) {
-  const size_t plane = (size_t)g.nf1 * g.nf2;
-  const int chunk = (int)std::max<size_t>(1, std::min<size_t>(nb, kMaxFftElems / plane));
-  for (int k0 = 0; k0 < nb; k0 += chunk) {
-    int n = std::min(chunk, nb - k0);
-    FftPlan fp;
-    CURIG_TRY(fft_plan(dev, g.nf1, g.nf2, n, sizeof(T) == 8, sign, &fp));
-    void *work = nullptr;
-    if (fp.work_bytes) CURIG_TRY(ws_get(dev, S_FFTW, fp.work_bytes, &work));
-    CURIG_TRY(fft_exec(fp, (v2_t<T> *)grid + (size_t)k0 * plane, work, st));
+    const Problem &pb = pbs[i];
+    CURIG_TRY(plan_geometry((int)nvalid, pb.nx, pb.ny, pb.fov, pb.freq, pb.eps, pb.sigma, pb.sign, wmin, wmax,
+                            pb.dbl, &tn, &cps[i].g, &cps[i].ko));
+    cps[i].nvalid = nvalid;
   }
   return kOk;
-}
-
-// rows whose first plane lies in the cyclic interval [lo, lo+len) -> up to two
-// record ranges (hoff[l] = first record of bin layer l, layers of Pbin planes)
-void source_ranges(const Geom &g, const std::vector<int> &hoff, int lo, int len, int &a0, int &a1, int &b0,
-                   int &b1) {
-  if (len > g.P) len = g.P;
-  const int first = lo / g.Pbin;
-  const int last_plane = lo + len - 1;
-  a0 = a1 = b0 = b1 = 0;
-  if (last_plane < g.P) {
-    a0 = hoff[first];
-    a1 = hoff[last_plane / g.Pbin + 1];
-  } else {
-    a0 = hoff[first];
-    a1 = hoff[g.nlayer];
-    int lastl = (last_plane - g.P) / g.Pbin;
-    if (lastl >= first) lastl = first - 1;  // the whole circle is covered
-    if (lastl >= 0) {
-      b0 = hoff[0];
-      b1 = hoff[lastl + 1];
-    }
-  }
-}
-
-int device_lds_bytes() {
-  static int cached = -1;
-  if (cached < 0) {
-    int dev = 0, v = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
-      v = 65536;
-    cached = v;
-  }
-  return cached;
 }
 
 // Twiddle tables tw[t] = e^{+2 pi i t / n} for n = nf1 and n = nf2 (FP64 values,
@@ -1387,114 +1321,280 @@ int allow_lds(K kernel, int bytes) {
   return kOk;
 }
 
-template <typename T, int E, int KWM>
-int launch_spread_row(const Rec<T> *recs, const int *off, cpx<T> *H, const cpx<T> *tw1, const Geom &g, int p0, int nb,
-                      hipStream_t st) {
-  CURIG_TRY(allow_lds(spread_row_fft_kernel<T, E, KWM>, g.lds_row_bytes));
-  spread_row_fft_kernel<T, E, KWM><<<dim3(g.nf2 / g.rpw, nb), g.rpw * g.nf1 / E, g.lds_row_bytes, st>>>(
-      recs, off, H, tw1, g, p0);
+template <typename T>
+int allow_kernels(const Geom &g) {
+  if (!g.fft_lds) return kOk;
+  CURIG_TRY(allow_lds(spread_row_fft_kernel<T, 16, 8>, g.lds_row_bytes));
+  CURIG_TRY(allow_lds(spread_row_fft_kernel<T, 16, 16>, g.lds_row_bytes));
+  CURIG_TRY(allow_lds(spread_row_fft_kernel<T, 16, kMaxKernelWidth>, g.lds_row_bytes));
+  CURIG_TRY(allow_lds(row_ifft_expand_kernel<T, 16>, g.lds_row_bytes));
+  const int l1 = fftc::padded_size(g.A2 * g.Wc1) * (int)sizeof(cpx<T>);
+  const int l2 = fftc::padded_size(g.B2 * g.Wc2) * (int)sizeof(cpx<T>);
+  CURIG_TRY(allow_lds(col_pass1_kernel<T, false>, l1));
+  CURIG_TRY(allow_lds(col_pass1_kernel<T, true>, l1));
+  CURIG_TRY(allow_lds(col_pass2_kernel<T, false>, l2));
+  CURIG_TRY(allow_lds(col_pass2_kernel<T, true>, l2));
   return kOk;
 }
 
-template <typename T, int E>
-int spread_row_dispatch(const Rec<T> *recs, const int *off, cpx<T> *H, const cpx<T> *tw1, const Geom &g, int p0,
-                        int nb, hipStream_t st) {
-  if (g.kw <= 8) return launch_spread_row<T, E, 8>(recs, off, H, tw1, g, p0, nb, st);
-  if (g.kw <= 16) return launch_spread_row<T, E, 16>(recs, off, H, tw1, g, p0, nb, st);
-  return launch_spread_row<T, E, kMaxKernelWidth>(recs, off, H, tw1, g, p0, nb, st);
+// FFT for a batch of nb planes, at most kMaxFftElems elements per rocFFT call
+// (the reference caps cuFFT batches the same way, MAX_CUFFT_ELEM, cugridder.cu:230-246;
+// a single call over 2^32 elements returns wrong planes).
+constexpr size_t kMaxFftElems = size_t(1) << 28;
+
+template <typename T>
+int batch_fft(int dev, const Geom &g, int nb, int sign, void *grid, hipStream_t st) {
+  const size_t plane = (size_t)g.nf1 * g.nf2;
+  const int chunk = (int)std::max<size_t>(1, std::min<size_t>(nb, kMaxFftElems / plane));
+  for (int k0 = 0; k0 < nb; k0 += chunk) {
+    int n = std::min(chunk, nb - k0);
+    FftPlan fp;
+    CURIG_TRY(fft_plan(dev, g.nf1, g.nf2, n, sizeof(T) == 8, sign, &fp));
+    void *work = nullptr;
+    if (fp.work_bytes) CURIG_TRY(ws_get(dev, S_FFTW, fp.work_bytes, &work));
+    CURIG_TRY(fft_exec(fp, (v2_t<T> *)grid + (size_t)k0 * plane, work, st));
+  }
+  return kOk;
+}
+
+// Phase 2 (one wait): allocate the working set for all problems, upload the
+// correction tables and twiddles, create FFT plans and kernel attributes, so
+// that phase 3 queues kernels only.
+template <typename T>
+int reserve(const std::vector<ChanPlan> &cps, const Problem *pbs, bool to_image, int dev, hipStream_t st, Bufs &b) {
+  const Geom &g0 = cps[0].g;
+  size_t nbins = 1, nscan = 1, nrow = 1, grid = 1, h = 1, npix = 1, nvalid = 1;
+  for (size_t i = 0; i < cps.size(); ++i) {
+    const Geom &g = cps[i].g;
+    if (g.kw != g0.kw || g.nf1 != g0.nf1 || g.nf2 != g0.nf2 || g.N1 != g0.N1 || g.N2 != g0.N2) {
+      set_error("channels of one call must share image size, epsilon and sigma");
+      return kErrArg;
+    }
+    size_t nb = (size_t)g.nlayer * g.ntiles + 1;
+    nbins = std::max(nbins, nb);
+    nscan = std::max(nscan, (nb + kScanTile - 1) / kScanTile + 1);
+    nrow = std::max(nrow, (size_t)std::max(pbs[i].nrow, 1));
+    nvalid = std::max(nvalid, (size_t)std::max<long>(cps[i].nvalid, 1));
+    const size_t plane = (size_t)g.nf1 * g.nf2, pix = (size_t)g.N1 * g.N2;
+    grid = std::max(grid, (to_image && g.fft_lds ? pix : plane) * g.Pb);
+    h = std::max(h, (size_t)g.nf2 * g.N1 * g.Pb);
+    npix = std::max(npix, pix);
+  }
+  void *vp;
+  CURIG_TRY(ws_get(dev, S_HIST, sizeof(int) * nbins, &vp));
+  b.hist = (int *)vp;
+  CURIG_TRY(ws_get(dev, S_OFF, sizeof(int) * nbins, &vp));
+  b.off = (int *)vp;
+  CURIG_TRY(ws_get(dev, S_BSUM, sizeof(int) * nscan, &vp));
+  b.bsum = (int *)vp;
+  CURIG_TRY(ws_get(dev, S_BINRANK, sizeof(int2) * nrow, &vp));
+  b.binrank = (int2 *)vp;
+  CURIG_TRY(ws_get(dev, S_RECS, sizeof(Rec<T>) * nrow, &vp));
+  b.recs = vp;
+  CURIG_TRY(ws_get(dev, S_PERM, sizeof(int) * nrow, &vp));
+  b.perm = (int *)vp;
+  CURIG_TRY(ws_get(dev, S_GRID, sizeof(v2_t<T>) * grid, &vp));
+  b.grid = vp;
+  CURIG_TRY(ws_get(dev, S_ACC, sizeof(double2) * npix, &vp));
+  b.acc = (double2 *)vp;
+  if (!to_image) {
+    CURIG_TRY(ws_get(dev, S_VACC, sizeof(double2) * nvalid, &vp));
+    b.vacc = (double2 *)vp;
+  }
+  if (g0.fft_lds) {
+    CURIG_TRY(ws_get(dev, S_H, sizeof(cpx<T>) * h, &vp));
+    b.H = vp;
+    if (!to_image) {
+      CURIG_TRY(ws_get(dev, S_H2, sizeof(cpx<T>) * h, &vp));
+      b.H2 = vp;
+    }
+    const cpx<T> *t1, *t2;
+    CURIG_TRY(twiddles<T>(dev, g0, st, t1, t2));
+    b.tw1 = t1;
+    b.tw2 = t2;
+    CURIG_TRY(allow_kernels<T>(g0));
+  } else {
+    // create every FFT plan the batches will use (full and tail batches, split
+    // into chunks of at most kMaxFftElems elements) and size the work buffer
+    for (const ChanPlan &cp : cps) {
+      const Geom &g = cp.g;
+      const size_t plane = (size_t)g.nf1 * g.nf2;
+      const int chunk = (int)std::max<size_t>(1, std::min<size_t>(g.Pb, kMaxFftElems / plane));
+      const int tail = g.P % g.Pb;
+      const int sizes[4] = {std::min(chunk, g.Pb), g.Pb % chunk, std::min(chunk, tail), tail % chunk};
+      for (int k = 0; k < 4; ++k) {
+        if (sizes[k] <= 0) continue;
+        FftPlan fp;
+        CURIG_TRY(fft_plan(dev, g.nf1, g.nf2, sizes[k], sizeof(T) == 8, to_image ? +1 : -1, &fp));
+        if (fp.work_bytes) CURIG_TRY(ws_get(dev, S_FFTW, fp.work_bytes, &vp));
+      }
+    }
+  }
+  // correction tables (shared by all channels: same kernel and grid sizes)
+  const KernelOpts &ko = cps[0].ko;
+  size_t n1 = g0.nf1 / 2 + 1, n2 = g0.nf2 / 2 + 1;
+  std::vector<double> ht(n1 + n2 + 2 * kMaxQuadNodes);
+  fseries_grid(ko, g0.nf1, ht.data());
+  fseries_grid(ko, g0.nf2, ht.data() + n1);
+  kernel_quadrature(ko, ht.data() + n1 + n2, ht.data() + n1 + n2 + g0.nquad);
+  CURIG_TRY(ws_get(dev, S_TAB, sizeof(double) * ht.size(), &vp));
+  b.tabs = (double *)vp;
+  CURIG_HIP(hipMemcpyAsync(b.tabs, ht.data(), sizeof(double) * ht.size(), hipMemcpyHostToDevice, st));
+  CURIG_HIP(hipStreamSynchronize(st));
+  b.dev = dev;
+  return kOk;
+}
+
+// Phase 3: binning of one problem (kernels only).
+template <typename T>
+void enqueue_binning(const Problem &pb, const ChanPlan &cp, const Bufs &b, hipStream_t st, bool with_value) {
+  const Geom &g = cp.g;
+  const long nbins = (long)g.nlayer * g.ntiles;
+  const long nscan = (nbins + kScanTile - 1) / kScanTile;
+  fill_int_kernel<<<nblocks(nbins), 256, 0, st>>>(b.hist, nbins, 0);
+  if (pb.nrow > 0)
+    bin_count_kernel<<<nblocks(pb.nrow), 256, 0, st>>>(pb.uvw, pb.nrow, pb.flag, pb.flag_stride, g, b.hist,
+                                                       b.binrank);
+  scan_reduce_kernel<<<(int)nscan, kScanBlock, 0, st>>>(b.hist, nbins, b.bsum);
+  scan_sums_kernel<<<1, 1024, 0, st>>>(b.bsum, (int)nscan);
+  scan_apply_kernel<<<(int)nscan, kScanBlock, 0, st>>>(b.hist, nbins, b.bsum, b.off, (int)cp.nvalid);
+  if (pb.nrow > 0)
+    bin_scatter_kernel<T><<<nblocks(pb.nrow), 256, 0, st>>>(pb.uvw, pb.nrow, pb.vis, pb.vis_stride, pb.wgt,
+                                                             pb.wgt_stride, g, b.binrank, b.off, (Rec<T> *)b.recs,
+                                                             b.perm, with_value ? 1 : 0);
+}
+
+template <typename T, int KWM>
+void launch_spread(const Rec<T> *recs, const int *off, v2_t<T> *grid, const Geom &g, int p0, int nb,
+                   hipStream_t st) {
+  dim3 grd(g.ntiles, nb);
+  size_t lds = sizeof(T) * 2 * g.T1 * g.T2;
+  spread_tile_kernel<T, KWM><<<grd, 256, lds, st>>>(recs, off, grid, g, p0);
+}
+
+template <typename T>
+void spread_dispatch(const Rec<T> *recs, const int *off, v2_t<T> *grid, const Geom &g, int p0, int nb,
+                     hipStream_t st) {
+  if (g.kw <= 8)
+    launch_spread<T, 8>(recs, off, grid, g, p0, nb, st);
+  else if (g.kw <= 16)
+    launch_spread<T, 16>(recs, off, grid, g, p0, nb, st);
+  else
+    launch_spread<T, kMaxKernelWidth>(recs, off, grid, g, p0, nb, st);
+}
+
+template <typename T>
+void interp_dispatch(const Rec<T> *recs, const int *off, const v2_t<T> *grid, double2 *vacc, const Geom &g,
+                     long nvalid, int p0, int nb, hipStream_t st) {
+  if (nvalid <= 0) return;
+  // the kernel reads its record ranges from the bin offsets; a batch touches at
+  // most the records of (nb + kw - 1) planes' layers, rounded up to whole layers
+  const double frac = std::min(1.0, (double)(nb + g.kw - 1 + 2 * g.Pbin) / g.P);
+  const int nbk = std::max(1, std::min(nblocks((long)(nvalid * frac) + 1), 4096));
+  if (g.kw <= 8)
+    interp_kernel<T, 8><<<nbk, 256, 0, st>>>(recs, off, grid, vacc, g, p0, nb);
+  else if (g.kw <= 16)
+    interp_kernel<T, 16><<<nbk, 256, 0, st>>>(recs, off, grid, vacc, g, p0, nb);
+  else
+    interp_kernel<T, kMaxKernelWidth><<<nbk, 256, 0, st>>>(recs, off, grid, vacc, g, p0, nb);
+}
+
+template <typename T, int KWM>
+void launch_spread_row(const Rec<T> *recs, const int *off, cpx<T> *H, const cpx<T> *tw1, const Geom &g, int p0,
+                       int nb, hipStream_t st) {
+  spread_row_fft_kernel<T, 16, KWM><<<dim3(g.nf2 / g.rpw, nb), g.rpw * g.nf1 / 16, g.lds_row_bytes, st>>>(
+      recs, off, H, tw1, g, p0);
 }
 
 // ms2dirty planes of one batch: spread + row FFT -> H, column passes -> compact Fc [nb][N2][N1]
 template <typename T>
-int lds_fft_forward(const Rec<T> *recs, const int *off, const Geom &g, int p0, int nb, cpx<T> *Fc, cpx<T> *H,
-                    const cpx<T> *tw1, const cpx<T> *tw2, hipStream_t st, Timer &tm, int &mark_fft) {
+void lds_fft_forward(const Rec<T> *recs, const int *off, const Geom &g, int p0, int nb, cpx<T> *Fc, cpx<T> *H,
+                     const cpx<T> *tw1, const cpx<T> *tw2, hipStream_t st, Timer &tm, int &mark_fft) {
   const int sign = +1;
-  CURIG_TRY((spread_row_dispatch<T, 16>(recs, off, H, tw1, g, p0, nb, st)));
+  if (g.kw <= 8)
+    launch_spread_row<T, 8>(recs, off, H, tw1, g, p0, nb, st);
+  else if (g.kw <= 16)
+    launch_spread_row<T, 16>(recs, off, H, tw1, g, p0, nb, st);
+  else
+    launch_spread_row<T, kMaxKernelWidth>(recs, off, H, tw1, g, p0, nb, st);
   mark_fft = tm.mark();
   const int ncb1 = (g.N1 + g.Wc1 - 1) / g.Wc1, ncb2 = (g.N1 + g.Wc2 - 1) / g.Wc2;
   const int l1 = fftc::padded_size(g.A2 * g.Wc1) * (int)sizeof(cpx<T>);
   const int l2 = fftc::padded_size(g.B2 * g.Wc2) * (int)sizeof(cpx<T>);
-  CURIG_TRY(allow_lds(col_pass1_kernel<T, false>, l1));
   col_pass1_kernel<T, false><<<dim3(g.B2 * ncb1, nb), g.A2 * g.Wc1 / 16, l1, st>>>(
       H, H, tw2, g.nf2, g.N1, g.N2, g.A2, g.B2, g.Wc1, sign);
-  CURIG_TRY(allow_lds(col_pass2_kernel<T, true>, l2));
   col_pass2_kernel<T, true><<<dim3(g.A2 * ncb2, nb), g.B2 * g.Wc2 / 16, l2, st>>>(
       H, Fc, tw2, g.nf2, g.N1, g.N2, g.A2, g.B2, g.Wc2, sign);
-  CURIG_HIP(hipGetLastError());
-  return kOk;
 }
 
 // dirty2ms planes of one batch: compact Fc [nb][N2][N1] (in G's buffer) -> full planes G
 template <typename T>
-int lds_fft_backward(const Geom &g, int nb, cpx<T> *G, cpx<T> *H, cpx<T> *H2, const cpx<T> *tw1,
-                     const cpx<T> *tw2, hipStream_t st) {
+void lds_fft_backward(const Geom &g, int nb, cpx<T> *G, cpx<T> *H, cpx<T> *H2, const cpx<T> *tw1,
+                      const cpx<T> *tw2, hipStream_t st) {
   const int sign = -1;
   const int ncb1 = (g.N1 + g.Wc1 - 1) / g.Wc1, ncb2 = (g.N1 + g.Wc2 - 1) / g.Wc2;
   const int l1 = fftc::padded_size(g.A2 * g.Wc1) * (int)sizeof(cpx<T>);
   const int l2 = fftc::padded_size(g.B2 * g.Wc2) * (int)sizeof(cpx<T>);
-  CURIG_TRY(allow_lds(col_pass1_kernel<T, true>, l1));
   col_pass1_kernel<T, true><<<dim3(g.B2 * ncb1, nb), g.A2 * g.Wc1 / 16, l1, st>>>(
       G, H, tw2, g.nf2, g.N1, g.N2, g.A2, g.B2, g.Wc1, sign);
-  CURIG_TRY(allow_lds(col_pass2_kernel<T, false>, l2));
   col_pass2_kernel<T, false><<<dim3(g.A2 * ncb2, nb), g.B2 * g.Wc2 / 16, l2, st>>>(
       H, H2, tw2, g.nf2, g.N1, g.N2, g.A2, g.B2, g.Wc2, sign);
-  CURIG_TRY(allow_lds(row_ifft_expand_kernel<T, 16>, g.lds_row_bytes));
   row_ifft_expand_kernel<T, 16><<<dim3(g.nf2 / g.rpw, nb), g.rpw * g.nf1 / 16, g.lds_row_bytes, st>>>(
       H2, G, tw1, g.nf1, g.nf2, g.N1, g.rpw, sign);
-  CURIG_HIP(hipGetLastError());
-  return kOk;
 }
 
+void fill_stats(Stats *sts, const ChanPlan &cp) {
+  const Geom &g = cp.g;
+  sts->kw = g.kw;
+  sts->nf1 = g.nf1;
+  sts->nf2 = g.nf2;
+  sts->P = g.P;
+  sts->Pb = g.Pb;
+  sts->nbatch = g.nbatch;
+  sts->T1 = g.T1;
+  sts->T2 = g.T2;
+  sts->fft_lds = g.fft_lds;
+  sts->nvis = cp.nvalid;
+  sts->beta = g.beta;
+  sts->gamma = g.gamma;
+  sts->C = g.C;
+  sts->D = g.D;
+}
+
+// Phase 3: queue one ms2dirty problem (no waits and no allocation unless timing).
 template <typename T>
-int ms2dirty_impl(const Problem &pb, hipStream_t st, const Tuning *tune, Stats *sts) {
-  CURIG_TRY(check_args(pb));
-  int dev = 0;
-  CURIG_HIP(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> lk(g_ws_mu);
+int enqueue_ms2dirty(const Problem &pb, const ChanPlan &cp, const Bufs &b, hipStream_t st, Stats *sts) {
+  const Geom &g = cp.g;
   Timer tm;
   tm.on = sts != nullptr;
   tm.s = st;
   int e0 = tm.mark();
-  Geom g;
-  KernelOpts ko;
-  Rec<T> *recs;
-  int *perm, *off, nrec;
-  double *tabs;
-  CURIG_TRY(prepare<T>(pb, st, tune, dev, g, ko, recs, perm, off, nrec, tabs, true));
+  enqueue_binning<T>(pb, cp, b, st, true);
+  CURIG_HIP(hipGetLastError());
   int e1 = tm.mark();
-  void *vp;
-  const size_t plane = (size_t)g.nf1 * g.nf2;
   const long npix = (long)g.N1 * g.N2;
-  CURIG_TRY(ws_get(dev, S_GRID, sizeof(v2_t<T>) * (g.fft_lds ? (size_t)npix : plane) * g.Pb, &vp));
-  v2_t<T> *grid = (v2_t<T> *)vp;
-  CURIG_TRY(ws_get(dev, S_ACC, sizeof(double2) * npix, &vp));
-  double2 *acc = (double2 *)vp;
-  const double *fk1 = tabs, *fk2 = tabs + (g.nf1 / 2 + 1), *q = fk2 + (g.nf2 / 2 + 1);
-  cpx<T> *H = nullptr;
-  const cpx<T> *tw1 = nullptr, *tw2 = nullptr;
-  if (g.fft_lds) {
-    CURIG_TRY(ws_get(dev, S_H, sizeof(cpx<T>) * (size_t)g.nf2 * g.N1 * g.Pb, &vp));
-    H = (cpx<T> *)vp;
-    CURIG_TRY(twiddles<T>(dev, g, st, tw1, tw2));
-  }
+  const Rec<T> *recs = (const Rec<T> *)b.recs;
+  v2_t<T> *grid = (v2_t<T> *)b.grid;
+  const double *fk1 = b.tabs, *fk2 = b.tabs + (g.nf1 / 2 + 1), *q = fk2 + (g.nf2 / 2 + 1);
   double t_spread = 0, t_fft = 0, t_wacc = 0;
-  for (int b = 0; b < g.nbatch; ++b) {
-    int p0 = b * g.Pb, nb = std::min(g.Pb, g.P - p0);
+  for (int bi = 0; bi < g.nbatch; ++bi) {
+    const int p0 = bi * g.Pb, nb = std::min(g.Pb, g.P - p0);
     int a = tm.mark(), bm;
     if (g.fft_lds) {
       // the fused kernel's time counts as spread (t_grid); the column passes as FFT
-      CURIG_TRY(lds_fft_forward<T>(recs, off, g, p0, nb, (cpx<T> *)grid, H, tw1, tw2, st, tm, bm));
+      lds_fft_forward<T>(recs, b.off, g, p0, nb, (cpx<T> *)grid, (cpx<T> *)b.H, (const cpx<T> *)b.tw1,
+                         (const cpx<T> *)b.tw2, st, tm, bm);
     } else {
-      spread_dispatch<T>(recs, off, grid, g, p0, nb, st);
-      CURIG_HIP(hipGetLastError());
+      spread_dispatch<T>(recs, b.off, grid, g, p0, nb, st);
       bm = tm.mark();
-      CURIG_TRY(batch_fft<T>(dev, g, nb, +1, grid, st));
+      CURIG_TRY(batch_fft<T>(b.dev, g, nb, +1, grid, st));
     }
+    CURIG_HIP(hipGetLastError());
     int c = tm.mark();
     if (g.fft_lds)
-      wacc_compact_kernel<T><<<nblocks(npix), 256, 0, st>>>((const cpx<T> *)grid, acc, g, p0, nb, b == 0 ? 1 : 0);
+      wacc_compact_kernel<T><<<nblocks(npix), 256, 0, st>>>((const cpx<T> *)grid, b.acc, g, p0, nb, bi == 0 ? 1 : 0);
     else
-      wacc_kernel<T><<<nblocks(npix), 256, 0, st>>>(grid, acc, g, p0, nb, b == 0 ? 1 : 0);
+      wacc_kernel<T><<<nblocks(npix), 256, 0, st>>>(grid, b.acc, g, p0, nb, bi == 0 ? 1 : 0);
     CURIG_HIP(hipGetLastError());
     int d = tm.mark();
     if (tm.on) {
@@ -1506,17 +1606,14 @@ int ms2dirty_impl(const Problem &pb, hipStream_t st, const Tuning *tune, Stats *
   }
   int e2 = tm.mark();
   if (pb.dbl)
-    epilogue_kernel<double><<<nblocks(npix), 256, 0, st>>>(acc, pb.dirty, g, fk1, fk2, q, pb.accumulate ? 1 : 0);
+    epilogue_kernel<double><<<nblocks(npix), 256, 0, st>>>(b.acc, pb.dirty, g, fk1, fk2, q, pb.accumulate ? 1 : 0);
   else
-    epilogue_kernel<float><<<nblocks(npix), 256, 0, st>>>(acc, pb.dirty, g, fk1, fk2, q, pb.accumulate ? 1 : 0);
+    epilogue_kernel<float><<<nblocks(npix), 256, 0, st>>>(b.acc, pb.dirty, g, fk1, fk2, q, pb.accumulate ? 1 : 0);
   CURIG_HIP(hipGetLastError());
   int e3 = tm.mark();
   if (sts) {
     CURIG_HIP(hipStreamSynchronize(st));
-    sts->kw = g.kw; sts->nf1 = g.nf1; sts->nf2 = g.nf2; sts->P = g.P; sts->Pb = g.Pb; sts->nbatch = g.nbatch;
-    sts->T1 = g.T1; sts->T2 = g.T2; sts->nvis = nrec; sts->beta = g.beta; sts->gamma = g.gamma;
-    sts->fft_lds = g.fft_lds;
-    sts->C = g.C; sts->D = g.D;
+    fill_stats(sts, cp);
     sts->t_prep = tm.ms(e0, e1);
     sts->t_grid = t_spread;
     sts->t_fft = t_fft;
@@ -1529,21 +1626,13 @@ int ms2dirty_impl(const Problem &pb, hipStream_t st, const Tuning *tune, Stats *
 }
 
 template <typename T>
-int dirty2ms_impl(const Problem &pb, hipStream_t st, const Tuning *tune, Stats *sts) {
-  CURIG_TRY(check_args(pb));
-  int dev = 0;
-  CURIG_HIP(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> lk(g_ws_mu);
+int enqueue_dirty2ms(const Problem &pb, const ChanPlan &cp, const Bufs &b, hipStream_t st, Stats *sts) {
+  const Geom &g = cp.g;
   Timer tm;
   tm.on = sts != nullptr;
   tm.s = st;
   int e0 = tm.mark();
-  Geom g;
-  KernelOpts ko;
-  Rec<T> *recs;
-  int *perm, *off, nrec;
-  double *tabs;
-  CURIG_TRY(prepare<T>(pb, st, tune, dev, g, ko, recs, perm, off, nrec, tabs, false));
+  enqueue_binning<T>(pb, cp, b, st, false);
   // every output row is written: flagged rows get 0
   if (pb.nrow > 0) {
     if (pb.dbl)
@@ -1551,45 +1640,22 @@ int dirty2ms_impl(const Problem &pb, hipStream_t st, const Tuning *tune, Stats *
     else
       zero_rows_kernel<float><<<nblocks(pb.nrow), 256, 0, st>>>(pb.vis, pb.vis_stride, pb.nrow);
   }
-  std::vector<int> hoff(g.nlayer + 1);
-  {
-    // first record of every bin layer, gathered on the device and copied once
-    void *hp;
-    CURIG_TRY(ws_get(dev, S_CNT, sizeof(int) * (g.nlayer + 1), &hp));
-    gather_layers_kernel<<<nblocks(g.nlayer + 1), 256, 0, st>>>(off, g.ntiles, g.nlayer, (int *)hp);
-    CURIG_HIP(hipGetLastError());
-    CURIG_HIP(hipMemcpyAsync(hoff.data(), hp, sizeof(int) * (g.nlayer + 1), hipMemcpyDeviceToHost, st));
-  }
-  void *vp;
-  const size_t plane = (size_t)g.nf1 * g.nf2;
-  CURIG_TRY(ws_get(dev, S_GRID, sizeof(v2_t<T>) * plane * g.Pb, &vp));
-  v2_t<T> *grid = (v2_t<T> *)vp;
   const long npix = (long)g.N1 * g.N2;
-  CURIG_TRY(ws_get(dev, S_ACC, sizeof(double2) * npix, &vp));
-  double2 *gimg = (double2 *)vp;
-  CURIG_TRY(ws_get(dev, S_VACC, sizeof(double2) * std::max(nrec, 1), &vp));
-  double2 *vacc = (double2 *)vp;
-  const double *fk1 = tabs, *fk2 = tabs + (g.nf1 / 2 + 1), *q = fk2 + (g.nf2 / 2 + 1);
+  const size_t plane = (size_t)g.nf1 * g.nf2;
+  const Rec<T> *recs = (const Rec<T> *)b.recs;
+  v2_t<T> *grid = (v2_t<T> *)b.grid;
+  double2 *gimg = b.acc;
+  const double *fk1 = b.tabs, *fk2 = b.tabs + (g.nf1 / 2 + 1), *q = fk2 + (g.nf2 / 2 + 1);
   if (pb.dbl)
     prologue_kernel<double><<<nblocks(npix), 256, 0, st>>>(pb.dirty, gimg, g, fk1, fk2, q);
   else
     prologue_kernel<float><<<nblocks(npix), 256, 0, st>>>(pb.dirty, gimg, g, fk1, fk2, q);
-  fill_zero_c128_kernel<<<nblocks(nrec), 256, 0, st>>>(vacc, nrec);
+  fill_zero_c128_kernel<<<nblocks(cp.nvalid), 256, 0, st>>>(b.vacc, cp.nvalid);
   CURIG_HIP(hipGetLastError());
-  CURIG_HIP(hipStreamSynchronize(st));  // hoff
   int e1 = tm.mark();
-  cpx<T> *H = nullptr, *H2 = nullptr;
-  const cpx<T> *tw1 = nullptr, *tw2 = nullptr;
-  if (g.fft_lds) {
-    CURIG_TRY(ws_get(dev, S_H, sizeof(cpx<T>) * (size_t)g.nf2 * g.N1 * g.Pb, &vp));
-    H = (cpx<T> *)vp;
-    CURIG_TRY(ws_get(dev, S_H2, sizeof(cpx<T>) * (size_t)g.nf2 * g.N1 * g.Pb, &vp));
-    H2 = (cpx<T> *)vp;
-    CURIG_TRY(twiddles<T>(dev, g, st, tw1, tw2));
-  }
   double t_interp = 0, t_fft = 0, t_fill = 0;
-  for (int b = 0; b < g.nbatch; ++b) {
-    int p0 = b * g.Pb, nb = std::min(g.Pb, g.P - p0);
+  for (int bi = 0; bi < g.nbatch; ++bi) {
+    const int p0 = bi * g.Pb, nb = std::min(g.Pb, g.P - p0);
     int a = tm.mark();
     if (g.fft_lds)
       wfill_compact_kernel<T><<<nblocks(npix), 256, 0, st>>>(gimg, (cpx<T> *)grid, g, p0, nb);
@@ -1598,13 +1664,13 @@ int dirty2ms_impl(const Problem &pb, hipStream_t st, const Tuning *tune, Stats *
     CURIG_HIP(hipGetLastError());
     int bm = tm.mark();
     if (g.fft_lds)
-      CURIG_TRY(lds_fft_backward<T>(g, nb, (cpx<T> *)grid, H, H2, tw1, tw2, st));
+      lds_fft_backward<T>(g, nb, (cpx<T> *)grid, (cpx<T> *)b.H, (cpx<T> *)b.H2, (const cpx<T> *)b.tw1,
+                          (const cpx<T> *)b.tw2, st);
     else
-      CURIG_TRY(batch_fft<T>(dev, g, nb, -1, grid, st));
+      CURIG_TRY(batch_fft<T>(b.dev, g, nb, -1, grid, st));
+    CURIG_HIP(hipGetLastError());
     int c = tm.mark();
-    int ra0, ra1, rb0, rb1;
-    source_ranges(g, hoff, imod_host(p0 - g.kw + 1, g.P), nb + g.kw - 1, ra0, ra1, rb0, rb1);
-    interp_dispatch<T>(recs, grid, vacc, g, p0, nb, ra0, ra1, rb0, rb1, st);
+    interp_dispatch<T>(recs, b.off, grid, b.vacc, g, cp.nvalid, p0, nb, st);
     CURIG_HIP(hipGetLastError());
     int d = tm.mark();
     if (tm.on) {
@@ -1615,22 +1681,19 @@ int dirty2ms_impl(const Problem &pb, hipStream_t st, const Tuning *tune, Stats *
     }
   }
   int e2 = tm.mark();
-  if (nrec > 0) {
+  if (cp.nvalid > 0) {
     if (pb.dbl)
-      finalize_kernel<double><<<nblocks(nrec), 256, 0, st>>>(perm, vacc, nrec, pb.uvw, pb.wgt, pb.wgt_stride,
-                                                              pb.vis, pb.vis_stride, g);
+      finalize_kernel<double><<<nblocks(cp.nvalid), 256, 0, st>>>(b.perm, b.vacc, (int)cp.nvalid, pb.uvw, pb.wgt,
+                                                                   pb.wgt_stride, pb.vis, pb.vis_stride, g);
     else
-      finalize_kernel<float><<<nblocks(nrec), 256, 0, st>>>(perm, vacc, nrec, pb.uvw, pb.wgt, pb.wgt_stride,
-                                                             pb.vis, pb.vis_stride, g);
+      finalize_kernel<float><<<nblocks(cp.nvalid), 256, 0, st>>>(b.perm, b.vacc, (int)cp.nvalid, pb.uvw, pb.wgt,
+                                                                  pb.wgt_stride, pb.vis, pb.vis_stride, g);
     CURIG_HIP(hipGetLastError());
   }
   int e3 = tm.mark();
   if (sts) {
     CURIG_HIP(hipStreamSynchronize(st));
-    sts->kw = g.kw; sts->nf1 = g.nf1; sts->nf2 = g.nf2; sts->P = g.P; sts->Pb = g.Pb; sts->nbatch = g.nbatch;
-    sts->T1 = g.T1; sts->T2 = g.T2; sts->nvis = nrec; sts->beta = g.beta; sts->gamma = g.gamma;
-    sts->fft_lds = g.fft_lds;
-    sts->C = g.C; sts->D = g.D;
+    fill_stats(sts, cp);
     sts->t_prep = tm.ms(e0, e1);
     sts->t_grid = t_interp;
     sts->t_fft = t_fft;
@@ -1642,14 +1705,129 @@ int dirty2ms_impl(const Problem &pb, hipStream_t st, const Tuning *tune, Stats *
   return kOk;
 }
 
+// Graph of a whole multi-channel call, kept while the same call repeats.
+struct GraphCache {
+  uint64_t key = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t cap = nullptr;
+};
+std::vector<GraphCache> g_graphs;
+
+uint64_t fnv(uint64_t h, const void *p, size_t n) {
+  const unsigned char *c = (const unsigned char *)p;
+  for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ull;
+  return h;
+}
+
+void drop_graph(GraphCache &gc) {
+  if (gc.exec) (void)hipGraphExecDestroy(gc.exec);
+  if (gc.graph) (void)hipGraphDestroy(gc.graph);
+  gc.exec = nullptr;
+  gc.graph = nullptr;
+  gc.key = 0;
+}
+
+template <typename T>
+int run_problems(bool to_image, const Problem *pbs, int n, hipStream_t st, const Tuning *tune, int graph_mode,
+                 Stats *sts) {
+  for (int i = 0; i < n; ++i) CURIG_TRY(check_args(pbs[i]));
+  int dev = 0;
+  CURIG_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  std::vector<ChanPlan> cps;
+  CURIG_TRY(plan_all(pbs, n, st, tune, dev, cps));
+  Bufs b;
+  CURIG_TRY((reserve<T>(cps, pbs, to_image, dev, st, b)));
+  // graph mode needs every kernel of the sequence to be a plain launch (LDS FFT path)
+  bool use_graph = graph_mode != 0;
+  for (const ChanPlan &cp : cps) use_graph = use_graph && cp.g.fft_lds;
+  if (!use_graph) {
+    Stats total, one;
+    for (int i = 0; i < n; ++i) {
+      CURIG_TRY(to_image ? enqueue_ms2dirty<T>(pbs[i], cps[i], b, st, sts ? &one : nullptr)
+                         : enqueue_dirty2ms<T>(pbs[i], cps[i], b, st, sts ? &one : nullptr));
+      if (sts) {
+        if (i == 0) total = one;
+        else {
+          total.nvis += one.nvis;
+          total.P += one.P;
+          total.t_prep += one.t_prep;
+          total.t_grid += one.t_grid;
+          total.t_fft += one.t_fft;
+          total.t_plane += one.t_plane;
+          total.t_final += one.t_final;
+          total.t_total += one.t_total;
+        }
+      }
+    }
+    if (sts) *sts = total;
+    return kOk;
+  }
+  if ((int)g_graphs.size() <= dev) g_graphs.resize(dev + 1);
+  GraphCache &gc = g_graphs[dev];
+  uint64_t key = 1469598103934665603ull;
+  key = fnv(key, &to_image, sizeof(to_image));
+  key = fnv(key, pbs, sizeof(Problem) * n);
+  for (const ChanPlan &cp : cps) key = fnv(key, &cp.g, sizeof(Geom));
+  key = fnv(key, &b, sizeof(Bufs));
+  Timer tm;
+  tm.on = sts != nullptr;
+  tm.s = st;
+  int e0 = tm.mark();
+  if (key != gc.key || !gc.exec) {
+    drop_graph(gc);
+    if (!gc.cap) CURIG_HIP(hipStreamCreateWithFlags(&gc.cap, hipStreamNonBlocking));
+    CURIG_HIP(hipStreamBeginCapture(gc.cap, hipStreamCaptureModeThreadLocal));
+    int ier = kOk;
+    for (int i = 0; i < n && ier == kOk; ++i)
+      ier = to_image ? enqueue_ms2dirty<T>(pbs[i], cps[i], b, gc.cap, nullptr)
+                     : enqueue_dirty2ms<T>(pbs[i], cps[i], b, gc.cap, nullptr);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(gc.cap, &graph);
+    if (ier != kOk) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return ier;
+    }
+    CURIG_HIP(e);
+    gc.graph = graph;
+    CURIG_HIP(hipGraphInstantiate(&gc.exec, gc.graph, nullptr, nullptr, 0));
+    gc.key = key;
+  }
+  CURIG_HIP(hipGraphLaunch(gc.exec, st));
+  int e1 = tm.mark();
+  if (sts) {
+    CURIG_HIP(hipStreamSynchronize(st));
+    fill_stats(sts, cps.back());
+    sts->nvis = 0;
+    sts->P = 0;
+    for (const ChanPlan &cp : cps) {
+      sts->nvis += cp.nvalid;
+      sts->P += cp.g.P;
+    }
+    sts->t_total = tm.ms(e0, e1);
+    sts->timed = 1;
+  }
+  return kOk;
+}
+
 }  // namespace
 
 int ms2dirty_device(const Problem &pb, hipStream_t st, const Tuning *tune, Stats *sts) {
-  return pb.dbl ? ms2dirty_impl<double>(pb, st, tune, sts) : ms2dirty_impl<float>(pb, st, tune, sts);
+  return pb.dbl ? run_problems<double>(true, &pb, 1, st, tune, 0, sts)
+                : run_problems<float>(true, &pb, 1, st, tune, 0, sts);
 }
 
 int dirty2ms_device(const Problem &pb, hipStream_t st, const Tuning *tune, Stats *sts) {
-  return pb.dbl ? dirty2ms_impl<double>(pb, st, tune, sts) : dirty2ms_impl<float>(pb, st, tune, sts);
+  return pb.dbl ? run_problems<double>(false, &pb, 1, st, tune, 0, sts)
+                : run_problems<float>(false, &pb, 1, st, tune, 0, sts);
+}
+
+int multi_device(bool to_image, const Problem *pbs, int n, hipStream_t st, const Tuning *tune, int graph_mode,
+                 Stats *sts) {
+  if (n <= 0) return kOk;
+  return pbs[0].dbl ? run_problems<double>(to_image, pbs, n, st, tune, graph_mode, sts)
+                    : run_problems<float>(to_image, pbs, n, st, tune, graph_mode, sts);
 }
 
 int explicit_ms2dirty_device(const Problem &pb, long npix, const int64_t *pix, void *out, hipStream_t st) {
@@ -1679,6 +1857,11 @@ void release_workspace() {
   std::lock_guard<std::mutex> lk(g_ws_mu);
   int cur = 0;
   (void)hipGetDevice(&cur);
+  for (size_t d = 0; d < g_graphs.size(); ++d) {
+    drop_graph(g_graphs[d]);
+    if (g_graphs[d].cap) (void)hipStreamDestroy(g_graphs[d].cap);
+  }
+  g_graphs.clear();
   for (size_t d = 0; d < g_ws.size(); ++d) {
     (void)hipSetDevice((int)d);
     for (int s = 0; s < S_NSLOT; ++s)

@@ -79,7 +79,12 @@ int curig_set_tuning(int tile_x, int tile_y, int planes_per_batch);
 /* FFT path of later calls: 0 = pruned LDS FFT where the grid is a power of two
  * and a row fits one workgroup's LDS (else rocFFT), 1 = always rocFFT. */
 int curig_set_fft_path(int path);
-/* 1: time the stages of later calls with hipEvents (adds synchronisation). */
+/* Multi-channel calls: 1 (default) = capture the whole per-channel sequence into
+ * one hipGraph and replay it while the same call repeats (power-of-two grids);
+ * 0 = queue the per-channel kernels directly. */
+int curig_set_graph_mode(int on);
+/* 1: time the stages of later calls with hipEvents (adds synchronisation;
+ * in graph mode only the total is timed). */
 int curig_enable_stats(int on);
 /* Stats of the last call on this process: ints[9] = kw, nf1, nf2, nf3, planes
  * per batch, batches, tile width, tile height, FFT path; reals[8] = rows gridded, t_prep,

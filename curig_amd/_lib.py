@@ -87,7 +87,7 @@ def load():
     lib.curig_kernel_fseries.restype = c_int
     lib.curig_gauss_legendre.argtypes = [c_int, _f64, _f64]
     lib.curig_gauss_legendre.restype = c_int
-    lib.curig_set_tuning.argtypes = [c_int, c_int, c_int]
+    lib.curig_set_tuning.argtypes = [c_int, c_int, c_int, c_int]
     lib.curig_set_tuning.restype = c_int
     lib.curig_set_fft_path.argtypes = [c_int]
     lib.curig_set_fft_path.restype = c_int

@@ -317,15 +317,17 @@ extern "C" int curig_gauss_legendre(int n, double *x, double *w) {
   return kOk;
 }
 
-extern "C" int curig_set_tuning(int tile_x, int tile_y, int planes_per_batch) {
-  if (tile_x < 0 || tile_y < 0 || planes_per_batch < 0 || tile_x > 256 || tile_y > 256 ||
-      (size_t)tile_x * tile_y > 4096) {
-    set_error("invalid tuning %d x %d, %d planes", tile_x, tile_y, planes_per_batch);
+extern "C" int curig_set_tuning(int tile_x, int tile_y, int planes_per_batch, int planes_per_bin) {
+  if (tile_x < 0 || tile_y < 0 || planes_per_batch < 0 || planes_per_bin < 0 || tile_x > 256 || tile_y > 256 ||
+      (size_t)tile_x * tile_y > 4096 || planes_per_batch > 4096 || planes_per_bin > 4096) {
+    set_error("invalid tuning %d x %d, %d planes per batch, %d per bin", tile_x, tile_y, planes_per_batch,
+              planes_per_bin);
     return kErrArg;
   }
   g_tuning.T1 = tile_x;
   g_tuning.T2 = tile_y;
   g_tuning.Pb = planes_per_batch;
+  g_tuning.Pbin = planes_per_bin;
   return kOk;
 }
 

@@ -73,9 +73,11 @@ int curig_kernel_fseries(double epsilon, double sigma, int nf, double *out);
 /* Host-only: n-point Gauss-Legendre rule on [-1, 1]. */
 int curig_gauss_legendre(int n, double *x, double *w);
 
-/* Tile width / height and w planes per batch used by later calls (0 = default).
- * The autotuner (curig_amd.autotune) sets these. */
-int curig_set_tuning(int tile_x, int tile_y, int planes_per_batch);
+/* Tuning of later calls (0 = default): tile width / height of the bins (for
+ * power-of-two grids only the height is used: bins are full-width row strips),
+ * w planes per batch (FFT batch) and w planes per bin layer.  The autotuner
+ * (curig_amd.autotune) sets these. */
+int curig_set_tuning(int tile_x, int tile_y, int planes_per_batch, int planes_per_bin);
 /* FFT path of later calls: 0 = pruned LDS FFT where the grid is a power of two
  * and a row fits one workgroup's LDS (else rocFFT), 1 = always rocFFT. */
 int curig_set_fft_path(int path);

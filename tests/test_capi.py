@@ -90,8 +90,8 @@ def test_error_codes_before_device_work():
         assert _call(fn, nx=0) == 3
         assert _call(fn, nrow=-1) == 4
     assert "epsilon" in lib.curig_last_error().decode() or lib.curig_last_error()
-    assert lib.curig_set_tuning(-1, 0, 0) == 5
-    assert lib.curig_set_tuning(0, 0, 0) == 0
+    assert lib.curig_set_tuning(-1, 0, 0, 0) == 5
+    assert lib.curig_set_tuning(0, 0, 0, 0) == 0
 
 
 def test_python_api_mirrors_reference_names():

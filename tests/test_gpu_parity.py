@@ -342,7 +342,7 @@ def test_small_plane_batches_match_restatement(planes_per_batch):
     restatement with every batch size."""
     lib = _lib()
     say(f"[plane batches] {planes_per_batch}")
-    assert lib.curig_set_tuning(0, 0, planes_per_batch) == 0
+    assert lib.curig_set_tuning(0, 0, planes_per_batch, 0) == 0
     try:
         uvw, vis, wgt, f0, rng = rand_case(77, 400, 48, 1.7, wscale=25.0)
         nx, ny, fov, eps = 48, 40, 12.0, 1e-7
@@ -354,7 +354,7 @@ def test_small_plane_batches_match_restatement(planes_per_batch):
         d32 = gpu_f32(True, uvw, vis, wgt, nx, ny, fov, f0, eps)
         assert O.l2error(R.ms2dirty(uvw, vis, wgt, nx, ny, fov, f0, eps), d32) < 3e-6
     finally:
-        lib.curig_set_tuning(0, 0, 0)
+        lib.curig_set_tuning(0, 0, 0, 0)
 
 
 @pytest.mark.parametrize("nx,ny", [(128, 128), (256, 128), (128, 512), (512, 512)])

@@ -99,7 +99,7 @@ struct Tuning {
   int Pb = 0;           // 0 -> from the memory budget
   int fft = 0;          // 0 auto (LDS path when it applies), 1 force rocFFT
   int lds_bytes = 0;    // LDS per workgroup of the device (0: 160 KiB, gfx950)
-  size_t grid_budget_bytes = size_t(24) << 30;
+  size_t grid_budget_bytes = size_t(48) << 30;
 };
 
 }  // namespace curig

@@ -747,6 +747,93 @@ __global__ void __launch_bounds__(256) explicit_kernel(const double *__restrict_
 // --------------------------------------------------------------------------
 using fftc::cpx;
 
+// Record ranges of one row workgroup (plane p, grid rows [row0, row0 + nrows)):
+// the bins of every layer holding a first plane in [p - kw + 1, p] and of every
+// strip overlapping rows [row0 - kw + 1, row0 + nrows - 1] (both cyclic), with
+// exclusive prefix counts, so that the workgroup walks them as one flat index
+// range and issues all its record loads at once.
+constexpr int kMaxWalk = 128;
+struct WalkTable {
+  int nb, total;
+  int start[kMaxWalk], pre[kMaxWalk];
+};
+
+// Number of tiles walk_tiles(a, len, n, t, ntile) visits.
+__host__ __device__ inline int tiles_count(int a, int len, int n, int t, int ntile) {
+  if (len > n) len = n;
+  int tile = a / t;
+  int covered = ((tile + 1) * t < n ? (tile + 1) * t : n) - a;
+  int visited = 1;
+  while (covered < len && visited < ntile) {
+    tile = (tile + 1 == ntile) ? 0 : tile + 1;
+    covered += ((tile + 1) * t < n ? (tile + 1) * t : n) - tile * t;
+    ++visited;
+  }
+  return visited;
+}
+
+// Upper bound of the bins a row workgroup walks (any p, row0).
+__host__ __device__ inline int walk_bins_bound(int kw, int Pbin, int nlayer, int nrows, int T2, int nty) {
+  int nl = (kw - 1) / Pbin + 2, ns = (nrows + kw - 2) / T2 + 2;
+  if (nl > nlayer) nl = nlayer;
+  if (ns > nty) ns = nty;
+  return nl * ns;
+}
+
+// Called by the 64 lanes of wave 0; the table is read after a barrier.
+__device__ __forceinline__ void walk_table_build(WalkTable &wt, const int *__restrict__ off, const Geom &g, int p,
+                                                 int row0, int nrows) {
+  const int lane = threadIdx.x;
+  const int al = imod(p - g.kw + 1, g.P), as = imod(row0 - g.kw + 1, g.nf2);
+  const int nl = tiles_count(al, g.kw, g.P, g.Pbin, g.nlayer);
+  const int ns = tiles_count(as, nrows + g.kw - 1, g.nf2, g.T2, g.nty);
+  int nb = nl * ns;
+  if (nb > kMaxWalk) nb = kMaxWalk;  // the plan keeps nb within kMaxWalk
+  const int l0 = al / g.Pbin, s0 = as / g.T2;
+  int carry = 0;
+  for (int base = 0; base < nb; base += 64) {
+    const int k = base + lane;
+    int st = 0, len = 0;
+    if (k < nb) {
+      const int j = k / ns, i = k - j * ns;
+      int layer = l0 + j, strip = s0 + i;
+      if (layer >= g.nlayer) layer -= g.nlayer;
+      if (strip >= g.nty) strip -= g.nty;
+      const int b = layer * g.ntiles + strip;
+      st = off[b];
+      len = off[b + 1] - st;
+    }
+    int x = len;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int y = __shfl_up(x, d, 64);
+      if (lane >= d) x += y;
+    }
+    if (k < nb) {
+      wt.start[k] = st;
+      wt.pre[k] = carry + x - len;
+    }
+    carry += __shfl(x, 63, 64);
+  }
+  if (lane == 0) {
+    wt.nb = nb;
+    wt.total = carry;
+  }
+}
+
+// Record index of flat position t (0 <= t < total): the last range starting at or before t.
+__device__ __forceinline__ int walk_record(const WalkTable &wt, int t) {
+  int lo = 0, hi = wt.nb - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (wt.pre[mid] <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return wt.start[lo] + (t - wt.pre[lo]);
+}
+
 template <typename T, int E, int MAXR = 16>
 __device__ __forceinline__ void lds_fft(cpx<T> *buf, const fftc::Layout &L, int sign,
                                         const cpx<T> *__restrict__ tw, int n_tw) {
@@ -790,98 +877,188 @@ __device__ __forceinline__ int grid_index(int i, int N, int nf) {
   return i >= h ? i - h : nf + i - h;
 }
 
+// Spread of one record into the workgroup's grid rows [row0, row0 + rpw) of plane p (LDS).
+template <typename T, int KWM>
+__device__ __forceinline__ void spread_rows(const Rec<T> &rc, T *ldsf, const Geom &g, int p, int row0, T beta,
+                                            T cc, T hw) {
+  const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2, rpw = g.rpw;
+  int k = p - rc.iw;
+  if (k < 0) k += g.P;
+  if (k >= kw) return;
+  int ry = rc.iy - row0;
+  if (ry < 0) ry += nf2;
+  if (!(ry < rpw || ry + kw - 1 >= nf2)) return;
+  const T vw = es_eval(rc.dw + (T)k, beta, cc, hw);
+  const T vre = rc.re * vw, vim = rc.im * vw;
+  T kx[KWM];
+#pragma unroll
+  for (int i = 0; i < KWM; ++i) kx[i] = (i < kw) ? es_eval(rc.dx + (T)i, beta, cc, hw) : (T)0;
+  for (int jj = 0; jj < kw; ++jj) {
+    int cy = ry + jj;
+    if (cy >= nf2) cy -= nf2;
+    if (cy >= rpw) continue;
+    const T ky = es_eval(rc.dy + (T)jj, beta, cc, hw);
+    const T are = vre * ky, aim = vim * ky;
+    const int base = cy * nf1;
+#pragma unroll
+    for (int i = 0; i < KWM; ++i) {
+      if (i < kw) {
+        int cx = rc.ix + i;
+        if (cx >= nf1) cx -= nf1;
+        const int a = 2 * fftc::pad(base + cx);
+        atomicAdd(ldsf + a, are * kx[i]);
+        atomicAdd(ldsf + a + 1, aim * kx[i]);
+      }
+    }
+  }
+}
+
+// Interpolation of one record from the workgroup's rows of plane p (LDS): its
+// partial sum over those rows, added to the record's FP64 accumulator.
+template <typename T, int KWM>
+__device__ __forceinline__ void interp_rows(const Rec<T> &rc, int idx, const cpx<T> *lds, const Geom &g, int p,
+                                            int row0, double2 *__restrict__ vacc, T beta, T cc, T hw) {
+  const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2, rpw = g.rpw;
+  int k = p - rc.iw;
+  if (k < 0) k += g.P;
+  if (k >= kw) return;
+  int ry = rc.iy - row0;
+  if (ry < 0) ry += nf2;
+  if (!(ry < rpw || ry + kw - 1 >= nf2)) return;
+  T kx[KWM];
+#pragma unroll
+  for (int i = 0; i < KWM; ++i) kx[i] = (i < kw) ? es_eval(rc.dx + (T)i, beta, cc, hw) : (T)0;
+  T sr = 0, si = 0;
+  for (int jj = 0; jj < kw; ++jj) {
+    int cy = ry + jj;
+    if (cy >= nf2) cy -= nf2;
+    if (cy >= rpw) continue;
+    const T ky = es_eval(rc.dy + (T)jj, beta, cc, hw);
+    const int base = cy * nf1;
+    T rr = 0, ri = 0;
+#pragma unroll
+    for (int i = 0; i < KWM; ++i) {
+      if (i < kw) {
+        int cx = rc.ix + i;
+        if (cx >= nf1) cx -= nf1;
+        const cpx<T> f = lds[fftc::pad(base + cx)];
+        rr += f.x * kx[i];
+        ri += f.y * kx[i];
+      }
+    }
+    sr += rr * ky;
+    si += ri * ky;
+  }
+  const T kz = es_eval(rc.dw + (T)k, beta, cc, hw);
+  atomicAdd(&vacc[idx].x, (double)(sr * kz));
+  atomicAdd(&vacc[idx].y, (double)(si * kz));
+}
+
+
 // ms2dirty: spread into full grid rows held in LDS, then the row FFT (length
 // nf1), keeping the N1 image columns: H [nb][nf2][N1].  One workgroup owns rpw
-// rows of one w plane; rows are binned into full-width strips of T2 rows.
+// rows of one w plane (rpw * nf1 = E * blockDim.x); rows are binned into
+// full-width strips of T2 rows.
 template <typename T, int E, int KWM>
 __global__ void __launch_bounds__(1024) spread_row_fft_kernel(const Rec<T> *__restrict__ recs,
                                                              const int *__restrict__ off, cpx<T> *__restrict__ H,
                                                              const cpx<T> *__restrict__ tw, Geom g, int p0) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ WalkTable wt;
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
   T *ldsf = reinterpret_cast<T *>(smem_raw);
-  const int rpw = g.rpw, nf1 = g.nf1, nf2 = g.nf2, nt = blockDim.x;
+  const int rpw = g.rpw, nf1 = g.nf1, nt = blockDim.x;
   const int row0 = blockIdx.x * rpw, pl = blockIdx.y, p = p0 + pl;
-  const int tot = rpw * nf1;
-  const int ntot = fftc::padded_size(tot);
+  const int ntot = fftc::padded_size(rpw * nf1);
+  if (threadIdx.x < 64) walk_table_build(wt, off, g, p, row0, rpw);
   for (int i = threadIdx.x; i < ntot; i += nt) lds[i] = fftc::mk((T)0, (T)0);
   __syncthreads();
-  const int kw = g.kw;
-  const T beta = (T)g.beta, cc = (T)g.c, hw = (T)(0.5 * kw);
-  walk_tiles(imod(p - kw + 1, g.P), kw, g.P, g.Pbin, g.nlayer, [&](int sl) {
-    walk_tiles(imod(row0 - kw + 1, nf2), rpw + kw - 1, nf2, g.T2, g.nty, [&](int sty) {
-      const int bin = sl * g.ntiles + sty;
-      const int r0 = off[bin], r1 = off[bin + 1];
-      for (int r = r0 + threadIdx.x; r < r1; r += nt) {
-        const Rec<T> rc = recs[r];
-        int k = p - rc.iw;
-        if (k < 0) k += g.P;
-        if (k >= kw) continue;
-        int ry = rc.iy - row0;
-        if (ry < 0) ry += nf2;
-        if (!(ry < rpw || ry + kw - 1 >= nf2)) continue;
-        const T vw = es_eval(rc.dw + (T)k, beta, cc, hw);
-        const T vre = rc.re * vw, vim = rc.im * vw;
-        T kx[KWM];
-#pragma unroll
-        for (int i = 0; i < KWM; ++i) kx[i] = (i < kw) ? es_eval(rc.dx + (T)i, beta, cc, hw) : (T)0;
-        for (int jj = 0; jj < kw; ++jj) {
-          int cy = ry + jj;
-          if (cy >= nf2) cy -= nf2;
-          if (cy >= rpw) continue;
-          const T ky = es_eval(rc.dy + (T)jj, beta, cc, hw);
-          const T are = vre * ky, aim = vim * ky;
-          const int base = cy * nf1;
-#pragma unroll
-          for (int i = 0; i < KWM; ++i) {
-            if (i < kw) {
-              int cx = rc.ix + i;
-              if (cx >= nf1) cx -= nf1;
-              const int a = 2 * fftc::pad(base + cx);
-              atomicAdd(ldsf + a, are * kx[i]);
-              atomicAdd(ldsf + a + 1, aim * kx[i]);
-            }
-          }
-        }
-      }
-    });
-  });
+  const T beta = (T)g.beta, cc = (T)g.c, hw = (T)(0.5 * g.kw);
+  const int total = wt.total;
+  for (int t0 = threadIdx.x; t0 < total; t0 += 2 * nt) {
+    const int t1 = t0 + nt;
+    const Rec<T> ra = recs[walk_record(wt, t0)];
+    Rec<T> rb;
+    if (t1 < total) rb = recs[walk_record(wt, t1)];
+    spread_rows<T, KWM>(ra, ldsf, g, p, row0, beta, cc, hw);
+    if (t1 < total) spread_rows<T, KWM>(rb, ldsf, g, p, row0, beta, cc, hw);
+  }
   __syncthreads();
   const fftc::Layout L{nf1, rpw, nf1, 1, false};
   lds_fft<T, E, 8>(lds, L, +1, tw, nf1);
-  cpx<T> *dst = H + ((size_t)pl * nf2 + row0) * g.N1;
-  for (int i = threadIdx.x; i < rpw * g.N1; i += nt) {
-    const int r = i / g.N1, col = i - r * g.N1;
-    dst[i] = lds[fftc::pad(r * nf1 + grid_index(col, g.N1, nf1))];
+  // pruned write: all LDS reads first, then the stores
+  const int N1 = g.N1, nout = rpw * N1;
+  cpx<T> *dst = H + ((size_t)pl * g.nf2 + row0) * N1;
+  cpx<T> o[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = threadIdx.x + e * nt;
+    if (i < nout) {
+      const int r = rpw == 1 ? 0 : i / N1, col = i - r * N1;
+      o[e] = lds[fftc::pad(r * nf1 + grid_index(col, N1, nf1))];
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = threadIdx.x + e * nt;
+    if (i < nout) dst[i] = o[e];
   }
 }
 
-// dirty2ms row pass: H2 [nb][nf2][N1] -> G [nb][nf2][nf1] (zero outside the image columns)
-template <typename T, int E>
-__global__ void __launch_bounds__(1024) row_ifft_expand_kernel(const cpx<T> *__restrict__ H2,
-                                                               cpx<T> *__restrict__ G,
-                                                               const cpx<T> *__restrict__ tw, int nf1, int nf2,
-                                                               int N1, int rpw, int sign) {
+// dirty2ms: row pass of the column-transformed planes H2 [nb][nf2][N1] (each
+// row expanded to nf1 cells, zero outside the image columns), inverse row FFT in
+// LDS, then interpolation of every record touching the workgroup's rows of
+// plane p: per record a partial sum, added to its FP64 accumulator (atomics),
+// so the full grid planes are never written.
+template <typename T, int E, int KWM>
+__global__ void __launch_bounds__(1024) row_ifft_interp_kernel(const cpx<T> *__restrict__ H2,
+                                                              const Rec<T> *__restrict__ recs,
+                                                              const int *__restrict__ off,
+                                                              double2 *__restrict__ vacc,
+                                                              const cpx<T> *__restrict__ tw, Geom g, int p0,
+                                                              int sign) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ WalkTable wt;
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
-  const int row0 = blockIdx.x * rpw, pl = blockIdx.y, nt = blockDim.x;
-  const cpx<T> *src = H2 + ((size_t)pl * nf2 + row0) * N1;
-  const int tot = rpw * nf1;
-  for (int i = threadIdx.x; i < tot; i += nt) {
-    const int r = i / nf1, k1 = i - r * nf1;
+  const int rpw = g.rpw, nf1 = g.nf1, N1 = g.N1, nt = blockDim.x;
+  const int row0 = blockIdx.x * rpw, pl = blockIdx.y, p = p0 + pl;
+  const cpx<T> *src = H2 + ((size_t)pl * g.nf2 + row0) * N1;
+  const int lg = 31 - __clz(nf1);
+  cpx<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = threadIdx.x + e * nt;  // < rpw * nf1 = E * nt
+    const int r = i >> lg, k1 = i & (nf1 - 1);
     const int col = image_index(k1, N1, nf1);
-    lds[fftc::pad(i)] = col >= 0 ? src[r * N1 + col] : fftc::mk((T)0, (T)0);
+    v[e] = col >= 0 ? src[r * N1 + col] : fftc::mk((T)0, (T)0);
   }
+  if (threadIdx.x < 64) walk_table_build(wt, off, g, p, row0, rpw);
+#pragma unroll
+  for (int e = 0; e < E; ++e) lds[fftc::pad(threadIdx.x + e * nt)] = v[e];
   __syncthreads();
+  const int total = wt.total;
+  // first record of this thread, loaded while the FFT runs
+  int idx0 = 0;
+  Rec<T> r0;
+  if (threadIdx.x < total) {
+    idx0 = walk_record(wt, threadIdx.x);
+    r0 = recs[idx0];
+  }
   const fftc::Layout L{nf1, rpw, nf1, 1, false};
   lds_fft<T, E, 8>(lds, L, sign, tw, nf1);
-  cpx<T> *dst = G + ((size_t)pl * nf2 + row0) * nf1;
-  for (int i = threadIdx.x; i < tot; i += nt) dst[i] = lds[fftc::pad(i)];
+  const T beta = (T)g.beta, cc = (T)g.c, hw = (T)(0.5 * g.kw);
+  if (threadIdx.x < total) interp_rows<T, KWM>(r0, idx0, lds, g, p, row0, vacc, beta, cc, hw);
+  for (int t = threadIdx.x + nt; t < total; t += nt) {
+    const int idx = walk_record(wt, t);
+    interp_rows<T, KWM>(recs[idx], idx, lds, g, p, row0, vacc, beta, cc, hw);
+  }
 }
 
 // column pass 1: length-A transforms over k2 = B ka + kb (fixed kb), then the
 // four-step twiddle w^{sign r kb}.  Source: H itself (ms2dirty, in place) or the
 // compact w-filled plane Fc [nb][N2][N1] (dirty2ms; rows outside the image are 0).
+// A * Wc = 16 * blockDim.x: each thread loads (and stores) 16 elements, all
+// loads issued before the first LDS store.
 template <typename T, bool FROM_COMPACT>
 __global__ void __launch_bounds__(256) col_pass1_kernel(const cpx<T> *src, cpx<T> *H,
                                                          const cpx<T> *__restrict__ tw2, int nf2, int N1, int N2,
@@ -890,29 +1067,37 @@ __global__ void __launch_bounds__(256) col_pass1_kernel(const cpx<T> *src, cpx<T
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
   const int kb = blockIdx.x % B, cb = blockIdx.x / B, pl = blockIdx.y, nt = blockDim.x;
   const int c0 = cb * Wc, wc = min(Wc, N1 - c0);
-  const int tot = A * Wc;
-  for (int i = threadIdx.x; i < tot; i += nt) {
-    const int ka = i / Wc, c = i - ka * Wc;
+  const int lw = 31 - __clz(Wc);
+  cpx<T> v[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const int i = threadIdx.x + e * nt;
+    const int ka = i >> lw, c = i & (Wc - 1);
     const int k2 = B * ka + kb;
-    cpx<T> v = fftc::mk((T)0, (T)0);
+    v[e] = fftc::mk((T)0, (T)0);
     if (c < wc) {
       if (FROM_COMPACT) {
         const int row = image_index(k2, N2, nf2);
-        if (row >= 0) v = src[((size_t)pl * N2 + row) * N1 + c0 + c];
+        if (row >= 0) v[e] = src[((size_t)pl * N2 + row) * N1 + c0 + c];
       } else {
-        v = src[((size_t)pl * nf2 + k2) * N1 + c0 + c];
+        v[e] = src[((size_t)pl * nf2 + k2) * N1 + c0 + c];
       }
     }
-    lds[fftc::pad(i)] = v;
   }
+#pragma unroll
+  for (int e = 0; e < 16; ++e) lds[fftc::pad(threadIdx.x + e * nt)] = v[e];
   __syncthreads();
   const fftc::Layout L{A, Wc, 1, Wc, true};
   lds_fft<T, 16>(lds, L, sign, tw2, nf2);
-  for (int i = threadIdx.x; i < tot; i += nt) {
-    const int r = i / Wc, c = i - r * Wc;
+#pragma unroll
+  for (int e = 0; e < 16; ++e) v[e] = lds[fftc::pad(threadIdx.x + e * nt)];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const int i = threadIdx.x + e * nt;
+    const int r = i >> lw, c = i & (Wc - 1);
     if (c < wc) {
       const cpx<T> w = tw2[r * kb];
-      H[((size_t)pl * nf2 + B * r + kb) * N1 + c0 + c] = fftc::mulw(lds[fftc::pad(i)], w.x, (T)sign * w.y);
+      H[((size_t)pl * nf2 + B * r + kb) * N1 + c0 + c] = fftc::mulw(v[e], w.x, (T)sign * w.y);
     }
   }
 }
@@ -927,23 +1112,32 @@ __global__ void __launch_bounds__(256) col_pass2_kernel(const cpx<T> *__restrict
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
   const int r = blockIdx.x % A, cb = blockIdx.x / A, pl = blockIdx.y, nt = blockDim.x;
   const int c0 = cb * Wc, wc = min(Wc, N1 - c0);
-  const int tot = B * Wc;
-  for (int i = threadIdx.x; i < tot; i += nt) {
-    const int kb = i / Wc, c = i - kb * Wc;
-    lds[fftc::pad(i)] = c < wc ? H[((size_t)pl * nf2 + B * r + kb) * N1 + c0 + c] : fftc::mk((T)0, (T)0);
+  const int lw = 31 - __clz(Wc);
+  cpx<T> v[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const int i = threadIdx.x + e * nt;
+    const int kb = i >> lw, c = i & (Wc - 1);
+    v[e] = c < wc ? H[((size_t)pl * nf2 + B * r + kb) * N1 + c0 + c] : fftc::mk((T)0, (T)0);
   }
+#pragma unroll
+  for (int e = 0; e < 16; ++e) lds[fftc::pad(threadIdx.x + e * nt)] = v[e];
   __syncthreads();
   const fftc::Layout L{B, Wc, 1, Wc, true};
   lds_fft<T, 16>(lds, L, sign, tw2, nf2);
-  for (int i = threadIdx.x; i < tot; i += nt) {
-    const int q = i / Wc, c = i - q * Wc;
+#pragma unroll
+  for (int e = 0; e < 16; ++e) v[e] = lds[fftc::pad(threadIdx.x + e * nt)];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const int i = threadIdx.x + e * nt;
+    const int q = i >> lw, c = i & (Wc - 1);
     if (c >= wc) continue;
     const int a2 = r + A * q;
     if (PRUNE) {
       const int row = image_index(a2, N2, nf2);
-      if (row >= 0) out[((size_t)pl * N2 + row) * N1 + c0 + c] = lds[fftc::pad(i)];
+      if (row >= 0) out[((size_t)pl * N2 + row) * N1 + c0 + c] = v[e];
     } else {
-      out[((size_t)pl * nf2 + a2) * N1 + c0 + c] = lds[fftc::pad(i)];
+      out[((size_t)pl * nf2 + a2) * N1 + c0 + c] = v[e];
     }
   }
 }
@@ -1153,14 +1347,6 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
     return kErrSize;
   }
   Tuning tn = tune ? *tune : Tuning();
-  size_t plane = (size_t)g.nf1 * g.nf2 * (dbl ? 16 : 8);
-  int Pb = tn.Pb;
-  if (Pb <= 0) {
-    Pb = (int)std::max<size_t>(1, tn.grid_budget_bytes / plane);
-    Pb = std::min(Pb, 32);
-  }
-  g.Pb = std::max(1, std::min(Pb, g.P));
-  g.nbatch = (g.P + g.Pb - 1) / g.Pb;
   g.Pbin = std::max(1, std::min(tn.Pbin > 0 ? tn.Pbin : 8, g.P));
   g.nlayer = (g.P + g.Pbin - 1) / g.Pbin;
   // pruned LDS FFT: power-of-two grids whose rows fit one workgroup's LDS
@@ -1177,14 +1363,26 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
     g.rpw = std::max(1, std::min(4096 / g.nf1, g.nf2));
     g.Wc1 = std::min(64, std::max(1, 4096 / g.A2));
     g.Wc2 = std::min(64, std::max(1, 4096 / g.B2));
+    // static LDS of the row kernels (bin walk table) comes on top of the row
     size_t rb = (size_t)fftc::padded_size(g.rpw * g.nf1) * cs;
     size_t c1 = (size_t)fftc::padded_size(g.A2 * g.Wc1) * cs, c2 = (size_t)fftc::padded_size(g.B2 * g.Wc2) * cs;
     g.Erow = 16;
-    if (rb <= lds_max && c1 <= lds_max && c2 <= lds_max && g.rpw * g.nf1 / g.Erow <= 1024) {
+    if (rb + 2048 <= lds_max && c1 <= lds_max && c2 <= lds_max && g.rpw * g.nf1 / g.Erow <= 1024) {
       g.fft_lds = 1;
       g.lds_row_bytes = (int)rb;
     }
   }
+  // planes per batch from the memory budget: full planes (rocFFT path) or, on
+  // the LDS path, the compact plane plus the one or two column-pass buffers
+  size_t per_plane = (size_t)g.nf1 * g.nf2 * cs;
+  if (g.fft_lds) per_plane = ((size_t)g.N1 * g.N2 + 2 * (size_t)g.nf2 * g.N1) * cs;
+  int Pb = tn.Pb;
+  if (Pb <= 0) {
+    Pb = (int)std::max<size_t>(1, tn.grid_budget_bytes / per_plane);
+    Pb = std::min(Pb, 32);
+  }
+  g.Pb = std::max(1, std::min(Pb, g.P));
+  g.nbatch = (g.P + g.Pb - 1) / g.Pb;
   // bins: full-width row strips for the LDS path (one workgroup owns grid rows),
   // T1 x T2 tiles for the tile-owner spread
   if (g.fft_lds) {
@@ -1196,6 +1394,18 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
   }
   g.ntx = (g.nf1 + g.T1 - 1) / g.T1;
   g.nty = (g.nf2 + g.T2 - 1) / g.T2;
+  if (g.fft_lds) {
+    // a row workgroup walks at most kMaxWalk bins: coarsen strips, then layers
+    while (walk_bins_bound(g.kw, g.Pbin, g.nlayer, g.rpw, g.T2, g.nty) > kMaxWalk) {
+      if (g.T2 < g.nf2) {
+        g.T2 = std::min(2 * g.T2, g.nf2);
+        g.nty = (g.nf2 + g.T2 - 1) / g.T2;
+      } else {
+        g.Pbin = std::min(2 * g.Pbin, g.P);
+        g.nlayer = (g.P + g.Pbin - 1) / g.Pbin;
+      }
+    }
+  }
   g.ntiles = g.ntx * g.nty;
   if ((double)g.nlayer * g.ntiles > 1.5e9) {
     set_error("too many bins (%d layers x %d tiles)", g.nlayer, g.ntiles);
@@ -1327,7 +1537,9 @@ int allow_kernels(const Geom &g) {
   CURIG_TRY(allow_lds(spread_row_fft_kernel<T, 16, 8>, g.lds_row_bytes));
   CURIG_TRY(allow_lds(spread_row_fft_kernel<T, 16, 16>, g.lds_row_bytes));
   CURIG_TRY(allow_lds(spread_row_fft_kernel<T, 16, kMaxKernelWidth>, g.lds_row_bytes));
-  CURIG_TRY(allow_lds(row_ifft_expand_kernel<T, 16>, g.lds_row_bytes));
+  CURIG_TRY(allow_lds(row_ifft_interp_kernel<T, 16, 8>, g.lds_row_bytes));
+  CURIG_TRY(allow_lds(row_ifft_interp_kernel<T, 16, 16>, g.lds_row_bytes));
+  CURIG_TRY(allow_lds(row_ifft_interp_kernel<T, 16, kMaxKernelWidth>, g.lds_row_bytes));
   const int l1 = fftc::padded_size(g.A2 * g.Wc1) * (int)sizeof(cpx<T>);
   const int l2 = fftc::padded_size(g.B2 * g.Wc2) * (int)sizeof(cpx<T>);
   CURIG_TRY(allow_lds(col_pass1_kernel<T, false>, l1));
@@ -1376,7 +1588,7 @@ int reserve(const std::vector<ChanPlan> &cps, const Problem *pbs, bool to_image,
     nrow = std::max(nrow, (size_t)std::max(pbs[i].nrow, 1));
     nvalid = std::max(nvalid, (size_t)std::max<long>(cps[i].nvalid, 1));
     const size_t plane = (size_t)g.nf1 * g.nf2, pix = (size_t)g.N1 * g.N2;
-    grid = std::max(grid, (to_image && g.fft_lds ? pix : plane) * g.Pb);
+    grid = std::max(grid, (g.fft_lds ? pix : plane) * g.Pb);
     h = std::max(h, (size_t)g.nf2 * g.N1 * g.Pb);
     npix = std::max(npix, pix);
   }
@@ -1527,10 +1739,10 @@ void lds_fft_forward(const Rec<T> *recs, const int *off, const Geom &g, int p0, 
       H, Fc, tw2, g.nf2, g.N1, g.N2, g.A2, g.B2, g.Wc2, sign);
 }
 
-// dirty2ms planes of one batch: compact Fc [nb][N2][N1] (in G's buffer) -> full planes G
+// dirty2ms planes of one batch, column passes: compact Fc [nb][N2][N1] (in the
+// grid buffer) -> H2 [nb][nf2][N1]
 template <typename T>
-void lds_fft_backward(const Geom &g, int nb, cpx<T> *G, cpx<T> *H, cpx<T> *H2, const cpx<T> *tw1,
-                      const cpx<T> *tw2, hipStream_t st) {
+void lds_fft_backward(const Geom &g, int nb, cpx<T> *G, cpx<T> *H, cpx<T> *H2, const cpx<T> *tw2, hipStream_t st) {
   const int sign = -1;
   const int ncb1 = (g.N1 + g.Wc1 - 1) / g.Wc1, ncb2 = (g.N1 + g.Wc2 - 1) / g.Wc2;
   const int l1 = fftc::padded_size(g.A2 * g.Wc1) * (int)sizeof(cpx<T>);
@@ -1539,8 +1751,21 @@ void lds_fft_backward(const Geom &g, int nb, cpx<T> *G, cpx<T> *H, cpx<T> *H2, c
       G, H, tw2, g.nf2, g.N1, g.N2, g.A2, g.B2, g.Wc1, sign);
   col_pass2_kernel<T, false><<<dim3(g.A2 * ncb2, nb), g.B2 * g.Wc2 / 16, l2, st>>>(
       H, H2, tw2, g.nf2, g.N1, g.N2, g.A2, g.B2, g.Wc2, sign);
-  row_ifft_expand_kernel<T, 16><<<dim3(g.nf2 / g.rpw, nb), g.rpw * g.nf1 / 16, g.lds_row_bytes, st>>>(
-      H2, G, tw1, g.nf1, g.nf2, g.N1, g.rpw, sign);
+}
+
+// dirty2ms planes of one batch, row pass fused with the interpolation
+template <typename T>
+void lds_row_interp(const Rec<T> *recs, const int *off, const cpx<T> *H2, double2 *vacc, const cpx<T> *tw1,
+                    const Geom &g, int p0, int nb, hipStream_t st) {
+  const dim3 grd(g.nf2 / g.rpw, nb);
+  const int nt = g.rpw * g.nf1 / 16;
+  if (g.kw <= 8)
+    row_ifft_interp_kernel<T, 16, 8><<<grd, nt, g.lds_row_bytes, st>>>(H2, recs, off, vacc, tw1, g, p0, -1);
+  else if (g.kw <= 16)
+    row_ifft_interp_kernel<T, 16, 16><<<grd, nt, g.lds_row_bytes, st>>>(H2, recs, off, vacc, tw1, g, p0, -1);
+  else
+    row_ifft_interp_kernel<T, 16, kMaxKernelWidth><<<grd, nt, g.lds_row_bytes, st>>>(H2, recs, off, vacc, tw1, g,
+                                                                                      p0, -1);
 }
 
 void fill_stats(Stats *sts, const ChanPlan &cp) {
@@ -1664,13 +1889,18 @@ int enqueue_dirty2ms(const Problem &pb, const ChanPlan &cp, const Bufs &b, hipSt
     CURIG_HIP(hipGetLastError());
     int bm = tm.mark();
     if (g.fft_lds)
-      lds_fft_backward<T>(g, nb, (cpx<T> *)grid, (cpx<T> *)b.H, (cpx<T> *)b.H2, (const cpx<T> *)b.tw1,
-                          (const cpx<T> *)b.tw2, st);
+      lds_fft_backward<T>(g, nb, (cpx<T> *)grid, (cpx<T> *)b.H, (cpx<T> *)b.H2, (const cpx<T> *)b.tw2, st);
     else
       CURIG_TRY(batch_fft<T>(b.dev, g, nb, -1, grid, st));
     CURIG_HIP(hipGetLastError());
     int c = tm.mark();
-    interp_dispatch<T>(recs, b.off, grid, b.vacc, g, cp.nvalid, p0, nb, st);
+    // the fused row iFFT + interpolation counts as interpolation (t_grid)
+    if (g.fft_lds) {
+      if (cp.nvalid > 0)
+        lds_row_interp<T>(recs, b.off, (const cpx<T> *)b.H2, b.vacc, (const cpx<T> *)b.tw1, g, p0, nb, st);
+    } else {
+      interp_dispatch<T>(recs, b.off, grid, b.vacc, g, cp.nvalid, p0, nb, st);
+    }
     CURIG_HIP(hipGetLastError());
     int d = tm.mark();
     if (tm.on) {

@@ -67,10 +67,12 @@ def candidates(info):
     """Candidate settings for the plan's FFT path."""
     out = []
     if info["fft_lds"]:
-        for ty in (4, 8, 16):
-            for pbin in (4, 8, 16):
-                for pb in (8, 16, 32):
-                    out.append(dict(tile_x=0, tile_y=ty, planes_per_batch=pb, planes_per_bin=pbin))
+        # default first (strips of 2 rows, 4-plane layers, budget-sized batches)
+        out.append(dict(tile_x=0, tile_y=2, planes_per_batch=0, planes_per_bin=4))
+        for ty in (1, 2, 4, 8):
+            for pbin in (2, 4, 8, 16):
+                if (ty, pbin) != (2, 4):
+                    out.append(dict(tile_x=0, tile_y=ty, planes_per_batch=0, planes_per_bin=pbin))
     else:
         for tx, ty in ((32, 32), (64, 32), (64, 16), (32, 16)):
             for pbin in (4, 8):

@@ -85,17 +85,17 @@ struct Geom {
   int nquad;          // number of quadrature nodes for phihat3
   // pruned LDS FFT path (power-of-two grids); fft_lds == 0 selects rocFFT
   int fft_lds;
-  int A2, B2;         // nf2 = A2 * B2 (four-step split of the column transform)
-  int Wc1, Wc2;       // columns per workgroup in the two column passes
-  int rpw;            // rows per workgroup in the row pass
-  int Erow;           // elements per thread in the row pass (16)
+  int rpw;            // grid rows per workgroup in the row pass (rpw * nf1 = 16 * threads)
+  int cpw;            // grid columns per workgroup in the column pass (cpw * nf2 = 16 * threads)
+  int nct;            // 16-column tiles of a plane's row-transformed layout (ceil(N1 / 16))
   int lds_row_bytes;  // dynamic LDS of the row-pass workgroup
+  int lds_col_bytes;  // dynamic LDS of the column-pass workgroup
 };
 
 // Tunables (autotuned per problem class; defaults are safe for any size)
 struct Tuning {
-  int T1 = 0, T2 = 0;   // 0 -> default for the precision (tile / row-strip size)
-  int Pbin = 0;         // planes per bin layer (0 -> 8)
+  int T1 = 0, T2 = 0;   // 0 -> default (tile: 64 x 32 FP32, 32 x 32 FP64; row strip: 2 rows)
+  int Pbin = 0;         // planes per bin layer (0 -> 4)
   int Pb = 0;           // 0 -> from the memory budget
   int fft = 0;          // 0 auto (LDS path when it applies), 1 force rocFFT
   int lds_bytes = 0;    // LDS per workgroup of the device (0: 160 KiB, gfx950)

@@ -164,6 +164,23 @@ FC_HD void stage_store(Buf buf, const cpx<T> *v, const Layout &L, int Ns, int ti
   }
 }
 
+// Twiddle table e^{+2 pi i t / n} (0 <= t < n) as the product of two short
+// tables, hi[t >> lb] = e^{2 pi i (t >> lb) 2^lb / n} and lo[t mod 2^lb] =
+// e^{2 pi i (t mod 2^lb) / n}, small enough for LDS (2 sqrt(n) entries); each
+// twiddle is one complex product of two correctly rounded values.
+template <typename T, typename Ptr> struct TwoLevel {
+  Ptr hi, lo;
+  int lb;
+  FC_HD cpx<T> operator[](int t) const { return mul(hi[t >> lb], lo[t & ((1 << lb) - 1)]); }
+};
+// lb for a length-n table: ceil(log2(n) / 2); hi has n >> lb entries, lo 2^lb
+FC_HD int twiddle_split(int n) {
+  int m = 0;
+  while ((1 << m) < n) ++m;
+  return (m + 1) / 2;
+}
+FC_HD int twiddle_entries(int n) { return (n >> twiddle_split(n)) + (1 << twiddle_split(n)); }
+
 // Radices used for a length-n transform: maxr, maxr, ..., then the remainder.
 FC_HD int stage_radix(int n, int Ns, int maxr = 16) { return (Ns * maxr <= n) ? maxr : n / Ns; }
 

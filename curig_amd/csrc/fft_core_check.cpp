@@ -25,9 +25,32 @@ template <typename T> struct Checked {
   }
 };
 
+template <typename T, int E, typename Tw>
+int run_stages(int n, int count, int seq_stride, int elem_stride, int seq_fast, int sign, double *data, int n_tw,
+               int maxr, const Tw &tw);
+
 template <typename T, int E>
 int run_e(int n, int count, int seq_stride, int elem_stride, int seq_fast, int sign, double *data, int n_tw,
-          int maxr) {
+          int maxr, int two_level) {
+  std::vector<cpx<T>> twv(n_tw);
+  for (int t = 0; t < n_tw; ++t) twv[t] = mk((T)std::cos(2 * M_PI * t / n_tw), (T)std::sin(2 * M_PI * t / n_tw));
+  if (!two_level) {
+    Checked<T> tw{twv.data(), (long)twv.size(), "twiddle"};
+    return run_stages<T, E>(n, count, seq_stride, elem_stride, seq_fast, sign, data, n_tw, maxr, tw);
+  }
+  // two-level table as the kernels hold it in LDS: hi (n_tw >> lb) | lo (2^lb)
+  const int lb = twiddle_split(n_tw), nhi = n_tw >> lb, nlo = 1 << lb;
+  std::vector<cpx<T>> hv(nhi), lv(nlo);
+  for (int j = 0; j < nhi; ++j)
+    hv[j] = mk((T)std::cos(2 * M_PI * ((double)j * nlo) / n_tw), (T)std::sin(2 * M_PI * ((double)j * nlo) / n_tw));
+  for (int j = 0; j < nlo; ++j) lv[j] = mk((T)std::cos(2 * M_PI * j / n_tw), (T)std::sin(2 * M_PI * j / n_tw));
+  TwoLevel<T, Checked<T>> tw{Checked<T>{hv.data(), nhi, "twiddle hi"}, Checked<T>{lv.data(), nlo, "twiddle lo"}, lb};
+  return run_stages<T, E>(n, count, seq_stride, elem_stride, seq_fast, sign, data, n_tw, maxr, tw);
+}
+
+template <typename T, int E, typename Tw>
+int run_stages(int n, int count, int seq_stride, int elem_stride, int seq_fast, int sign, double *data, int n_tw,
+               int maxr, const Tw &tw) {
   Layout L{n, count, seq_stride, elem_stride, seq_fast != 0};
   const int total = n * count;
   if (total % E) return 1;
@@ -41,9 +64,6 @@ int run_e(int n, int count, int seq_stride, int elem_stride, int seq_fast, int s
       int i = s * seq_stride + k * elem_stride;
       buf[pad(i)] = mk((T)data[2 * (s * n + k)], (T)data[2 * (s * n + k) + 1]);
     }
-  std::vector<cpx<T>> twv(n_tw);
-  for (int t = 0; t < n_tw; ++t) twv[t] = mk((T)std::cos(2 * M_PI * t / n_tw), (T)std::sin(2 * M_PI * t / n_tw));
-  Checked<T> tw{twv.data(), (long)twv.size(), "twiddle"};
   std::vector<cpx<T>> regs((size_t)nthreads * E);
   for (int Ns = 1; Ns < n;) {
     int R = stage_radix(n, Ns, maxr);
@@ -79,12 +99,17 @@ int run_e(int n, int count, int seq_stride, int elem_stride, int seq_fast, int s
 
 }  // namespace
 
-extern "C" int fftc_check_run(int n, int count, int seq_stride, int elem_stride, int seq_fast, int sign,
-                              int dbl, double *data, int n_tw, int E, int maxr) {
+extern "C" int fftc_check_run2(int n, int count, int seq_stride, int elem_stride, int seq_fast, int sign,
+                               int dbl, double *data, int n_tw, int E, int maxr, int two_level) {
   if (n < 16 || (n & (n - 1)) || (maxr != 8 && maxr != 16)) return 3;
   if (E == 32)
-    return dbl ? run_e<double, 32>(n, count, seq_stride, elem_stride, seq_fast, sign, data, n_tw, maxr)
-               : run_e<float, 32>(n, count, seq_stride, elem_stride, seq_fast, sign, data, n_tw, maxr);
-  return dbl ? run_e<double, 16>(n, count, seq_stride, elem_stride, seq_fast, sign, data, n_tw, maxr)
-             : run_e<float, 16>(n, count, seq_stride, elem_stride, seq_fast, sign, data, n_tw, maxr);
+    return dbl ? run_e<double, 32>(n, count, seq_stride, elem_stride, seq_fast, sign, data, n_tw, maxr, two_level)
+               : run_e<float, 32>(n, count, seq_stride, elem_stride, seq_fast, sign, data, n_tw, maxr, two_level);
+  return dbl ? run_e<double, 16>(n, count, seq_stride, elem_stride, seq_fast, sign, data, n_tw, maxr, two_level)
+             : run_e<float, 16>(n, count, seq_stride, elem_stride, seq_fast, sign, data, n_tw, maxr, two_level);
+}
+
+extern "C" int fftc_check_run(int n, int count, int seq_stride, int elem_stride, int seq_fast, int sign,
+                              int dbl, double *data, int n_tw, int E, int maxr) {
+  return fftc_check_run2(n, count, seq_stride, elem_stride, seq_fast, sign, dbl, data, n_tw, E, maxr, 0);
 }

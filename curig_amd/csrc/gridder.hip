@@ -29,6 +29,7 @@
 #include "curig_internal.h"
 #include "fft.h"
 #include "fft_core.h"
+#include "trig.h"
 #include "gridder.h"
 
 namespace curig {
@@ -511,18 +512,19 @@ __device__ __forceinline__ void pixel_correction(int row, int col, const Geom &g
   phase = (nm1 - g.D) * g.C;
 }
 
+// acc row-major [N2][N1], or column-major [N1][N2] (LDS path) when colmajor
 template <typename TO>
 __global__ void __launch_bounds__(256) epilogue_kernel(const double2 *__restrict__ acc, void *__restrict__ out,
                                                         Geom g, const double *__restrict__ fk1,
                                                         const double *__restrict__ fk2,
-                                                        const double *__restrict__ q, int accumulate) {
+                                                        const double *__restrict__ q, int accumulate, int colmajor) {
   const long npix = (long)g.N1 * g.N2;
   for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < npix; idx += (long)gridDim.x * blockDim.x) {
     int row = (int)(idx / g.N1), col = (int)(idx - (long)row * g.N1);
     double scale, phase, s, c;
     pixel_correction(row, col, g, fk1, fk2, q, scale, phase);
     sincos(phase, &s, &c);
-    double2 a = acc[idx];
+    double2 a = acc[colmajor ? (long)col * g.N2 + row : idx];
     double re = (a.x * c - a.y * s) * scale, im = (a.x * s + a.y * c) * scale;
     if (accumulate)
       add_c<TO>(out, idx, re, im);
@@ -737,13 +739,15 @@ __global__ void __launch_bounds__(256) explicit_kernel(const double *__restrict_
 // --------------------------------------------------------------------------
 // pruned LDS FFT path (power-of-two grids)
 //
-// ms2dirty per plane: row pass (length nf1, keep the N1 columns of the image)
-// -> column pass 1 (length-A2 transforms over rows spaced B2 apart, twiddle)
-// -> column pass 2 (length-B2 transforms over consecutive rows, keep the N2
-// image rows) -> compact plane [N2][N1] in image order.  dirty2ms runs the
-// mirror image: compact w-filled plane -> column passes (all nf2 outputs) ->
-// row pass that expands each row to nf1 cells.  nf2 = A2 * B2; a column
-// element k2 = B2 ka + kb, an output a2 = r + A2 q (four-step FFT).
+// ms2dirty per batch of planes: (1) one workgroup per grid row of a plane
+// spreads the row's visibilities into LDS, runs the row FFT (length nf1) and
+// writes the N1 image columns to the column-tiled buffer H; (2) one workgroup
+// per image column runs the column FFTs (length nf2) of all the batch's planes
+// and accumulates the N2 image rows with their w phases in FP64.  dirty2ms runs
+// the mirror image: (1) per image column, w-fill + inverse column FFT of every
+// plane -> H; (2) per grid row, inverse row FFT (the row expanded to nf1 cells)
+// and interpolation.  Each plane crosses HBM twice (written once, read once) at
+// half the grid size.
 // --------------------------------------------------------------------------
 using fftc::cpx;
 
@@ -834,9 +838,18 @@ __device__ __forceinline__ int walk_record(const WalkTable &wt, int t) {
   return wt.start[lo] + (t - wt.pre[lo]);
 }
 
-template <typename T, int E, int MAXR = 16>
-__device__ __forceinline__ void lds_fft(cpx<T> *buf, const fftc::Layout &L, int sign,
-                                        const cpx<T> *__restrict__ tw, int n_tw) {
+// Copies a two-level twiddle table (hi | lo, fftc::twiddle_entries(n) entries)
+// from global memory to LDS at dst; the caller's next barrier publishes it.
+template <typename T>
+__device__ __forceinline__ fftc::TwoLevel<T, const cpx<T> *> lds_twiddles(cpx<T> *dst,
+                                                                         const cpx<T> *__restrict__ src, int n) {
+  const int lb = fftc::twiddle_split(n), nhi = n >> lb, cnt = nhi + (1 << lb);
+  for (int i = threadIdx.x; i < cnt; i += blockDim.x) dst[i] = src[i];
+  return fftc::TwoLevel<T, const cpx<T> *>{dst, dst + nhi, lb};
+}
+
+template <typename T, int E, int MAXR = 16, typename Tw>
+__device__ __forceinline__ void lds_fft(cpx<T> *buf, const fftc::Layout &L, int sign, const Tw tw, int n_tw) {
   cpx<T> v[E];
   const int nt = blockDim.x;
   for (int Ns = 1; Ns < L.n;) {
@@ -864,6 +877,42 @@ __device__ __forceinline__ void lds_fft(cpx<T> *buf, const fftc::Layout &L, int 
       fftc::stage_store<T, E, 2>(buf, v, L, Ns, tid, nt);
     __syncthreads();
     Ns *= R;
+  }
+}
+
+// Stockham stages of `count` contiguous length-N sequences (buffer index
+// pad(s * N + k)) with N a compile-time constant: every stage's radix, stride
+// and twiddle step fold to constants, and the element indexing to shifts.
+template <typename T, int E, int MAXR, int N, int NS, typename Tw>
+__device__ __forceinline__ void fixed_stages(cpx<T> *buf, int count, int sign, const Tw &tw) {
+  if constexpr (NS < N) {
+    constexpr int R = (NS * MAXR <= N) ? MAXR : N / NS;
+    const fftc::Layout L{N, count, N, 1, false};
+    const int nt = blockDim.x;
+    int tid = threadIdx.x;
+    __asm__ volatile("" : "+v"(tid));
+    cpx<T> v[E];
+    fftc::stage_load<T, E, R>(buf, v, L, NS, sign, tw, N, tid, nt);
+    __syncthreads();
+    fftc::stage_store<T, E, R>(buf, v, L, NS, tid, nt);
+    __syncthreads();
+    fixed_stages<T, E, MAXR, N, NS * R>(buf, count, sign, tw);
+  }
+}
+
+// FFT of `count` contiguous sequences of length n = 2^lg (6 <= lg <= 14) in LDS.
+template <typename T, int E, int MAXR, typename Tw>
+__device__ __forceinline__ void lds_fft_rows(cpx<T> *buf, int lg, int count, int sign, const Tw &tw) {
+  switch (lg) {
+    case 6: fixed_stages<T, E, MAXR, 64, 1>(buf, count, sign, tw); break;
+    case 7: fixed_stages<T, E, MAXR, 128, 1>(buf, count, sign, tw); break;
+    case 8: fixed_stages<T, E, MAXR, 256, 1>(buf, count, sign, tw); break;
+    case 9: fixed_stages<T, E, MAXR, 512, 1>(buf, count, sign, tw); break;
+    case 10: fixed_stages<T, E, MAXR, 1024, 1>(buf, count, sign, tw); break;
+    case 11: fixed_stages<T, E, MAXR, 2048, 1>(buf, count, sign, tw); break;
+    case 12: fixed_stages<T, E, MAXR, 4096, 1>(buf, count, sign, tw); break;
+    case 13: fixed_stages<T, E, MAXR, 8192, 1>(buf, count, sign, tw); break;
+    default: fixed_stages<T, E, MAXR, 16384, 1>(buf, count, sign, tw); break;
   }
 }
 
@@ -955,14 +1004,32 @@ __device__ __forceinline__ void interp_rows(const Rec<T> &rc, int idx, const cpx
 }
 
 
+// Row-transformed planes live in a column-tiled layout: element (plane k, grid
+// row k2, image column col) at ((k * nct + col / 16) * nf2 + k2) * 16 + col % 16,
+// so a row's 16-column chunks are contiguous (row kernels) and a column's
+// elements are 16 apart (column kernels; the 16 columns of a tile run on one
+// XCD, whose L2 merges their accesses into whole lines).
+constexpr int kTile = 16;
+
+// Workgroup id -> work item such that each XCD (workgroups are dispatched to
+// the 8 XCDs round robin) takes one contiguous range of items.
+__device__ __forceinline__ int xcd_group(int bid, int n) {
+  const int q = n >> 3, r = n & 7, x = bid & 7, k = bid >> 3;
+  return x * q + (x < r ? x : r) + k;
+}
+
+template <typename T> struct MaxThreads { static constexpr int value = sizeof(T) == 8 ? 512 : 1024; };
+
 // ms2dirty: spread into full grid rows held in LDS, then the row FFT (length
-// nf1), keeping the N1 image columns: H [nb][nf2][N1].  One workgroup owns rpw
-// rows of one w plane (rpw * nf1 = E * blockDim.x); rows are binned into
-// full-width strips of T2 rows.
+// nf1), keeping the N1 image columns, written to the column-tiled layout H.  One
+// workgroup owns rpw rows of one w plane (rpw * nf1 = E * blockDim.x); rows are
+// binned into full-width strips of T2 rows.
 template <typename T, int E, int KWM>
-__global__ void __launch_bounds__(1024) spread_row_fft_kernel(const Rec<T> *__restrict__ recs,
-                                                             const int *__restrict__ off, cpx<T> *__restrict__ H,
-                                                             const cpx<T> *__restrict__ tw, Geom g, int p0) {
+__global__ void __launch_bounds__(MaxThreads<T>::value) spread_row_fft_kernel(const Rec<T> *__restrict__ recs,
+                                                                             const int *__restrict__ off,
+                                                                             cpx<T> *__restrict__ H,
+                                                                             const cpx<T> *__restrict__ tw, Geom g,
+                                                                             int p0) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ WalkTable wt;
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
@@ -970,6 +1037,7 @@ __global__ void __launch_bounds__(1024) spread_row_fft_kernel(const Rec<T> *__re
   const int rpw = g.rpw, nf1 = g.nf1, nt = blockDim.x;
   const int row0 = blockIdx.x * rpw, pl = blockIdx.y, p = p0 + pl;
   const int ntot = fftc::padded_size(rpw * nf1);
+  const auto tv = lds_twiddles<T>(lds + ntot, tw, nf1);
   if (threadIdx.x < 64) walk_table_build(wt, off, g, p, row0, rpw);
   for (int i = threadIdx.x; i < ntot; i += nt) lds[i] = fftc::mk((T)0, (T)0);
   __syncthreads();
@@ -984,54 +1052,62 @@ __global__ void __launch_bounds__(1024) spread_row_fft_kernel(const Rec<T> *__re
     if (t1 < total) spread_rows<T, KWM>(rb, ldsf, g, p, row0, beta, cc, hw);
   }
   __syncthreads();
-  const fftc::Layout L{nf1, rpw, nf1, 1, false};
-  lds_fft<T, E, 8>(lds, L, +1, tw, nf1);
-  // pruned write: all LDS reads first, then the stores
-  const int N1 = g.N1, nout = rpw * N1;
-  cpx<T> *dst = H + ((size_t)pl * g.nf2 + row0) * N1;
+  lds_fft_rows<T, E, 16>(lds, 31 - __clz(nf1), rpw, +1, tv);
+  // pruned write: all LDS reads first, then the stores (16-column chunks);
+  // offsets from the row's base in 32 bits (a plane holds < 2^31 elements)
+  const int N1 = g.N1, nout = rpw * N1, h1 = N1 / 2;
+  const int lgt = 35 - __clz(g.nf2);  // log2(nf2 * 16)
+  cpx<T> *hb = H + ((size_t)pl * g.nct * g.nf2 + row0) * kTile;
+  int tid = threadIdx.x;
+  __asm__ volatile("" : "+v"(tid));
   cpx<T> o[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const int i = threadIdx.x + e * nt;
+    const int i = tid + e * nt;
     if (i < nout) {
       const int r = rpw == 1 ? 0 : i / N1, col = i - r * N1;
-      o[e] = lds[fftc::pad(r * nf1 + grid_index(col, N1, nf1))];
+      const int k1 = col >= h1 ? col - h1 : nf1 + col - h1;
+      o[e] = lds[fftc::pad(r * nf1 + k1)];
     }
   }
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const int i = threadIdx.x + e * nt;
-    if (i < nout) dst[i] = o[e];
+    const int i = tid + e * nt;
+    if (i < nout) {
+      const int r = rpw == 1 ? 0 : i / N1, col = i - r * N1;
+      hb[((col >> 4) << lgt) + (r << 4) + (col & 15)] = o[e];
+    }
   }
 }
 
-// dirty2ms: row pass of the column-transformed planes H2 [nb][nf2][N1] (each
-// row expanded to nf1 cells, zero outside the image columns), inverse row FFT in
+// dirty2ms: row pass of the column-transformed planes (column-tiled H, each row
+// expanded to nf1 cells, zero outside the image columns), inverse row FFT in
 // LDS, then interpolation of every record touching the workgroup's rows of
 // plane p: per record a partial sum, added to its FP64 accumulator (atomics),
 // so the full grid planes are never written.
 template <typename T, int E, int KWM>
-__global__ void __launch_bounds__(1024) row_ifft_interp_kernel(const cpx<T> *__restrict__ H2,
-                                                              const Rec<T> *__restrict__ recs,
-                                                              const int *__restrict__ off,
-                                                              double2 *__restrict__ vacc,
-                                                              const cpx<T> *__restrict__ tw, Geom g, int p0,
-                                                              int sign) {
+__global__ void __launch_bounds__(MaxThreads<T>::value) row_ifft_interp_kernel(const cpx<T> *__restrict__ H,
+                                                                              const Rec<T> *__restrict__ recs,
+                                                                              const int *__restrict__ off,
+                                                                              double2 *__restrict__ vacc,
+                                                                              const cpx<T> *__restrict__ tw,
+                                                                              Geom g, int p0, int sign) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ WalkTable wt;
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
   const int rpw = g.rpw, nf1 = g.nf1, N1 = g.N1, nt = blockDim.x;
   const int row0 = blockIdx.x * rpw, pl = blockIdx.y, p = p0 + pl;
-  const cpx<T> *src = H2 + ((size_t)pl * g.nf2 + row0) * N1;
-  const int lg = 31 - __clz(nf1);
+  const int lg = 31 - __clz(nf1), lgt = 35 - __clz(g.nf2);
+  const cpx<T> *hb = H + ((size_t)pl * g.nct * g.nf2 + row0) * kTile;
   cpx<T> v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int i = threadIdx.x + e * nt;  // < rpw * nf1 = E * nt
     const int r = i >> lg, k1 = i & (nf1 - 1);
     const int col = image_index(k1, N1, nf1);
-    v[e] = col >= 0 ? src[r * N1 + col] : fftc::mk((T)0, (T)0);
+    v[e] = col >= 0 ? hb[((col >> 4) << lgt) + (r << 4) + (col & 15)] : fftc::mk((T)0, (T)0);
   }
+  const auto tv = lds_twiddles<T>(lds + fftc::padded_size(rpw * nf1), tw, nf1);
   if (threadIdx.x < 64) walk_table_build(wt, off, g, p, row0, rpw);
 #pragma unroll
   for (int e = 0; e < E; ++e) lds[fftc::pad(threadIdx.x + e * nt)] = v[e];
@@ -1044,8 +1120,7 @@ __global__ void __launch_bounds__(1024) row_ifft_interp_kernel(const cpx<T> *__r
     idx0 = walk_record(wt, threadIdx.x);
     r0 = recs[idx0];
   }
-  const fftc::Layout L{nf1, rpw, nf1, 1, false};
-  lds_fft<T, E, 8>(lds, L, sign, tw, nf1);
+  lds_fft_rows<T, E, 16>(lds, lg, rpw, sign, tv);
   const T beta = (T)g.beta, cc = (T)g.c, hw = (T)(0.5 * g.kw);
   if (threadIdx.x < total) interp_rows<T, KWM>(r0, idx0, lds, g, p, row0, vacc, beta, cc, hw);
   for (int t = threadIdx.x + nt; t < total; t += nt) {
@@ -1054,146 +1129,184 @@ __global__ void __launch_bounds__(1024) row_ifft_interp_kernel(const cpx<T> *__r
   }
 }
 
-// column pass 1: length-A transforms over k2 = B ka + kb (fixed kb), then the
-// four-step twiddle w^{sign r kb}.  Source: H itself (ms2dirty, in place) or the
-// compact w-filled plane Fc [nb][N2][N1] (dirty2ms; rows outside the image are 0).
-// A * Wc = 16 * blockDim.x: each thread loads (and stores) 16 elements, all
-// loads issued before the first LDS store.
-template <typename T, bool FROM_COMPACT>
-__global__ void __launch_bounds__(256) col_pass1_kernel(const cpx<T> *src, cpx<T> *H,
-                                                         const cpx<T> *__restrict__ tw2, int nf2, int N1, int N2,
-                                                         int A, int B, int Wc, int sign) {
+// z' (n - 1 - D) gamma h of image pixel (row, col), cuft.cu:53-92 pre-stage
+__device__ __forceinline__ double pixel_zp(int row, int col, const Geom &g) {
+  double nm1, n;
+  pixel_n(row, col, g, nm1, n);
+  return (nm1 - g.D) * g.gamma * g.h;
+}
+
+// ms2dirty column pass fused with the w-accumulation (w_term_dft, cuft.cu:520-572):
+// a workgroup owns cpw grid columns (cpw * nf2 = E * blockDim.x) and walks the
+// batch's nb planes: load the row-transformed columns, FFT over nf2, then for
+// the N2 image rows accT[col][row] += F_p e^{+i z' (p - P/2)}: the phase
+// argument and its reduction in FP64 (sin / cos of the reduced argument at the
+// planes' precision) and the sum over planes in FP64 registers.  Image rows per thread:
+// cpw * N2 <= 8 * blockDim.x (the plan requires 2 N2 <= nf2).
+template <typename T, int E>
+__global__ void __launch_bounds__(MaxThreads<T>::value) col_fwd_kernel(const cpx<T> *__restrict__ H,
+                                                                      double2 *__restrict__ accT,
+                                                                      const cpx<T> *__restrict__ tw, Geom g,
+                                                                      int p0, int nb, int first) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
-  const int kb = blockIdx.x % B, cb = blockIdx.x / B, pl = blockIdx.y, nt = blockDim.x;
-  const int c0 = cb * Wc, wc = min(Wc, N1 - c0);
-  const int lw = 31 - __clz(Wc);
-  cpx<T> v[16];
+  const int cpw = g.cpw, nf2 = g.nf2, N1 = g.N1, N2 = g.N2, nt = blockDim.x;
+  const int col0 = xcd_group(blockIdx.x, gridDim.x) * cpw;
+  const int lg = 31 - __clz(nf2), lgt = lg + 4;
+  const int nout = cpw * N2;
+  double ar[8], ai[8], zp[8];
 #pragma unroll
-  for (int e = 0; e < 16; ++e) {
-    const int i = threadIdx.x + e * nt;
-    const int ka = i >> lw, c = i & (Wc - 1);
-    const int k2 = B * ka + kb;
-    v[e] = fftc::mk((T)0, (T)0);
-    if (c < wc) {
-      if (FROM_COMPACT) {
-        const int row = image_index(k2, N2, nf2);
-        if (row >= 0) v[e] = src[((size_t)pl * N2 + row) * N1 + c0 + c];
-      } else {
-        v[e] = src[((size_t)pl * nf2 + k2) * N1 + c0 + c];
+  for (int e = 0; e < 8; ++e) {
+    ar[e] = ai[e] = zp[e] = 0.0;
+    const int o = threadIdx.x + e * nt;
+    if (o < nout) {
+      const int ci = o / N2, row = o - ci * N2;
+      if (col0 + ci < N1) zp[e] = pixel_zp(row, col0 + ci, g);
+    }
+  }
+  const auto tv = lds_twiddles<T>(lds + fftc::padded_size(cpw * nf2), tw, nf2);
+  for (int k = 0; k < nb; ++k) {
+    // opaque copy of the thread id: the per-plane addresses are formed inside
+    // the loop rather than held in registers across it
+    int tid = threadIdx.x;
+    __asm__ volatile("" : "+v"(tid));
+    const cpx<T> *hk = H + (size_t)k * g.nct * nf2 * kTile;
+    cpx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = tid + e * nt;
+      const int ci = i >> lg, k2 = i & (nf2 - 1), col = col0 + ci;
+      v[e] = col < N1 ? hk[((col >> 4) << lgt) + (k2 << 4) + (col & 15)] : fftc::mk((T)0, (T)0);
+    }
+    __syncthreads();  // the previous plane's LDS reads are done
+#pragma unroll
+    for (int e = 0; e < E; ++e) lds[fftc::pad(tid + e * nt)] = v[e];
+    __syncthreads();
+    lds_fft_rows<T, E, 16>(lds, lg, cpw, +1, tv);
+    const double m = (double)(p0 + k - g.P / 2);
+    int tid2 = threadIdx.x;
+    __asm__ volatile("" : "+v"(tid2));
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int o = tid2 + e * nt;
+      if (o < nout) {
+        const int ci = o / N2, row = o - ci * N2;
+        const cpx<T> f = lds[fftc::pad(ci * nf2 + grid_index(row, N2, nf2))];
+        double sn, cs;
+        plane_phase<T>(zp[e] * m, &sn, &cs);
+        const double fr = (double)f.x, fi = (double)f.y;
+        ar[e] += fr * cs - fi * sn;
+        ai[e] += fr * sn + fi * cs;
+      }
+      // one element's phase at a time (bounds the live FP64 temporaries)
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int o = threadIdx.x + e * nt;
+    if (o < nout) {
+      const int ci = o / N2, row = o - ci * N2, col = col0 + ci;
+      if (col < N1) {
+        const size_t a = (size_t)col * N2 + row;
+        double2 r = make_double2(ar[e], ai[e]);
+        if (!first) {
+          const double2 q = accT[a];
+          r.x += q.x;
+          r.y += q.y;
+        }
+        accT[a] = r;
       }
     }
   }
-#pragma unroll
-  for (int e = 0; e < 16; ++e) lds[fftc::pad(threadIdx.x + e * nt)] = v[e];
-  __syncthreads();
-  const fftc::Layout L{A, Wc, 1, Wc, true};
-  lds_fft<T, 16>(lds, L, sign, tw2, nf2);
-#pragma unroll
-  for (int e = 0; e < 16; ++e) v[e] = lds[fftc::pad(threadIdx.x + e * nt)];
-#pragma unroll
-  for (int e = 0; e < 16; ++e) {
-    const int i = threadIdx.x + e * nt;
-    const int r = i >> lw, c = i & (Wc - 1);
-    if (c < wc) {
-      const cpx<T> w = tw2[r * kb];
-      H[((size_t)pl * nf2 + B * r + kb) * N1 + c0 + c] = fftc::mulw(v[e], w.x, (T)sign * w.y);
-    }
-  }
 }
 
-// column pass 2: length-B transforms over k2 = B r + kb (fixed r); output a2 = r + A q.
-// PRUNE (ms2dirty): keep the N2 image rows, compact [nb][N2][N1].  Otherwise all nf2 rows.
-template <typename T, bool PRUNE>
-__global__ void __launch_bounds__(256) col_pass2_kernel(const cpx<T> *__restrict__ H, cpx<T> *__restrict__ out,
-                                                         const cpx<T> *__restrict__ tw2, int nf2, int N1, int N2,
-                                                         int A, int B, int Wc, int sign) {
+// dirty2ms column pass fused with the w-fill (w_term_idft, cuft.cu:575-612): a
+// workgroup owns cpw grid columns and, for each of the batch's nb planes, places
+// gT[col][row] e^{-i z' (p - P/2)} (FP64 phase argument) at the image rows of its columns
+// (zeros elsewhere), runs the inverse FFT over nf2 and writes the full columns
+// to the column-tiled layout H.
+template <typename T, int E>
+__global__ void __launch_bounds__(MaxThreads<T>::value) col_inv_kernel(const cpx<T> *__restrict__ gT,
+                                                                      cpx<T> *__restrict__ H,
+                                                                      const cpx<T> *__restrict__ tw, Geom g,
+                                                                      int p0, int nb) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
-  const int r = blockIdx.x % A, cb = blockIdx.x / A, pl = blockIdx.y, nt = blockDim.x;
-  const int c0 = cb * Wc, wc = min(Wc, N1 - c0);
-  const int lw = 31 - __clz(Wc);
-  cpx<T> v[16];
+  const int cpw = g.cpw, nf2 = g.nf2, N1 = g.N1, N2 = g.N2, nt = blockDim.x;
+  const int col0 = xcd_group(blockIdx.x, gridDim.x) * cpw;
+  const int lg = 31 - __clz(nf2), lgt = lg + 4;
+  const int nout = cpw * N2, nzero = cpw * (nf2 - N2), h2 = N2 / 2;
+  cpx<T> gv[8];
+  double zp[8];
 #pragma unroll
-  for (int e = 0; e < 16; ++e) {
-    const int i = threadIdx.x + e * nt;
-    const int kb = i >> lw, c = i & (Wc - 1);
-    v[e] = c < wc ? H[((size_t)pl * nf2 + B * r + kb) * N1 + c0 + c] : fftc::mk((T)0, (T)0);
+  for (int e = 0; e < 8; ++e) {
+    gv[e] = fftc::mk((T)0, (T)0);
+    zp[e] = 0.0;
+    const int o = threadIdx.x + e * nt;
+    if (o < nout) {
+      const int ci = o / N2, row = o - ci * N2, col = col0 + ci;
+      if (col < N1) {
+        gv[e] = gT[(size_t)col * N2 + row];
+        zp[e] = pixel_zp(row, col, g);
+      }
+    }
   }
+  const auto tv = lds_twiddles<T>(lds + fftc::padded_size(cpw * nf2), tw, nf2);
+  for (int k = 0; k < nb; ++k) {
+    const double m = (double)(p0 + k - g.P / 2);
+    int tid = threadIdx.x;
+    __asm__ volatile("" : "+v"(tid));
+    __syncthreads();  // the previous plane's LDS reads are done
 #pragma unroll
-  for (int e = 0; e < 16; ++e) lds[fftc::pad(threadIdx.x + e * nt)] = v[e];
-  __syncthreads();
-  const fftc::Layout L{B, Wc, 1, Wc, true};
-  lds_fft<T, 16>(lds, L, sign, tw2, nf2);
+    for (int e = 0; e < 8; ++e) {
+      const int o = tid + e * nt;
+      if (o < nout) {
+        const int ci = o / N2, row = o - ci * N2;
+        double sn, cs;
+        plane_phase<T>(-zp[e] * m, &sn, &cs);
+        const double gr = (double)gv[e].x, gi = (double)gv[e].y;
+        lds[fftc::pad(ci * nf2 + grid_index(row, N2, nf2))] =
+            fftc::mk((T)(gr * cs - gi * sn), (T)(gr * sn + gi * cs));
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    for (int z = tid; z < nzero; z += nt) {
+      const int ci = z / (nf2 - N2), j = z - ci * (nf2 - N2);
+      lds[fftc::pad(ci * nf2 + N2 - h2 + j)] = fftc::mk((T)0, (T)0);
+    }
+    __syncthreads();
+    lds_fft_rows<T, E, 16>(lds, lg, cpw, -1, tv);
+    int tid2 = threadIdx.x;
+    __asm__ volatile("" : "+v"(tid2));
+    cpx<T> v[E];
 #pragma unroll
-  for (int e = 0; e < 16; ++e) v[e] = lds[fftc::pad(threadIdx.x + e * nt)];
+    for (int e = 0; e < E; ++e) v[e] = lds[fftc::pad(tid2 + e * nt)];
+    cpx<T> *hk = H + (size_t)k * g.nct * nf2 * kTile;
 #pragma unroll
-  for (int e = 0; e < 16; ++e) {
-    const int i = threadIdx.x + e * nt;
-    const int q = i >> lw, c = i & (Wc - 1);
-    if (c >= wc) continue;
-    const int a2 = r + A * q;
-    if (PRUNE) {
-      const int row = image_index(a2, N2, nf2);
-      if (row >= 0) out[((size_t)pl * N2 + row) * N1 + c0 + c] = v[e];
-    } else {
-      out[((size_t)pl * nf2 + a2) * N1 + c0 + c] = v[e];
+    for (int e = 0; e < E; ++e) {
+      const int i = tid2 + e * nt;
+      const int ci = i >> lg, k2 = i & (nf2 - 1), col = col0 + ci;
+      if (col < N1) hk[((col >> 4) << lgt) + (k2 << 4) + (col & 15)] = v[e];
     }
   }
 }
 
-// dirty2ms w-fill into compact planes: Fc[k][pix] = g[pix] e^{-i z'(p0 + k - P/2)}
-template <typename T>
-__global__ void __launch_bounds__(256) wfill_compact_kernel(const double2 *__restrict__ gimg, cpx<T> *__restrict__ Fc,
-                                                             Geom g, int p0, int nb) {
+// dirty2ms prologue for the LDS path: gT[col][row] (precision T, column-major) =
+// dirty / n e^{-i C (z - D)} / (phihat1 phihat2 phihat3)
+template <typename TI, typename T>
+__global__ void __launch_bounds__(256) prologue_cols_kernel(const void *__restrict__ dirty, cpx<T> *__restrict__ gT,
+                                                             Geom g, const double *__restrict__ fk1,
+                                                             const double *__restrict__ fk2,
+                                                             const double *__restrict__ q) {
   const long npix = (long)g.N1 * g.N2;
   for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < npix; idx += (long)gridDim.x * blockDim.x) {
-    const int row = (int)(idx / g.N1), col = (int)(idx - (long)row * g.N1);
-    const double2 v = gimg[idx];
-    double nm1, n;
-    pixel_n(row, col, g, nm1, n);
-    const double zp = (nm1 - g.D) * g.gamma * g.h;
-    double s, c, ss, cs;
-    sincos(-zp * (double)(p0 - g.P / 2), &s, &c);
-    sincos(-zp, &ss, &cs);
-    for (int k = 0; k < nb; ++k) {
-      Fc[(size_t)k * npix + idx] = fftc::mk((T)(v.x * c - v.y * s), (T)(v.x * s + v.y * c));
-      const double cn = c * cs - s * ss;
-      s = s * cs + c * ss;
-      c = cn;
-    }
-  }
-}
-
-// w-accumulate over compact planes Fc [nb][N2][N1] (ms2dirty, LDS FFT path)
-template <typename T>
-__global__ void __launch_bounds__(256) wacc_compact_kernel(const cpx<T> *__restrict__ Fc, double2 *__restrict__ acc,
-                                                            Geom g, int p0, int nb, int first) {
-  const long npix = (long)g.N1 * g.N2;
-  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < npix; idx += (long)gridDim.x * blockDim.x) {
-    const int row = (int)(idx / g.N1), col = (int)(idx - (long)row * g.N1);
-    double nm1, n;
-    pixel_n(row, col, g, nm1, n);
-    const double zp = (nm1 - g.D) * g.gamma * g.h;
-    double s, c, ss, cs;
-    sincos(zp * (double)(p0 - g.P / 2), &s, &c);
-    sincos(zp, &ss, &cs);
-    double ar = 0.0, ai = 0.0;
-    for (int k = 0; k < nb; ++k) {
-      const cpx<T> f = Fc[(size_t)k * npix + idx];
-      const double fr = (double)f.x, fi = (double)f.y;
-      ar += fr * c - fi * s;
-      ai += fr * s + fi * c;
-      const double cn = c * cs - s * ss;
-      s = s * cs + c * ss;
-      c = cn;
-    }
-    if (first) {
-      acc[idx] = make_double2(ar, ai);
-    } else {
-      const double2 o = acc[idx];
-      acc[idx] = make_double2(o.x + ar, o.y + ai);
-    }
+    const int col = (int)(idx / g.N2), row = (int)(idx - (long)col * g.N2);
+    double scale, phase, s, c;
+    pixel_correction(row, col, g, fk1, fk2, q, scale, phase);
+    sincos(-phase, &s, &c);
+    const double2 a = load_c<TI>(dirty, (long)row * g.N1 + col);
+    gT[idx] = fftc::mk((T)((a.x * c - a.y * s) * scale), (T)((a.x * s + a.y * c) * scale));
   }
 }
 
@@ -1205,7 +1318,7 @@ namespace {
 // Grow-only device buffers, one set per device (calls on one device are serialised).
 enum Slot {
   S_PART, S_CNT, S_HIST, S_OFF, S_BSUM, S_BINRANK, S_RECS, S_PERM, S_GRID, S_ACC, S_VACC, S_TAB, S_FFTW,
-  S_H, S_H2, S_TW,
+  S_H, S_TW,
   S_NSLOT
 };
 struct Workspace {
@@ -1347,35 +1460,37 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
     return kErrSize;
   }
   Tuning tn = tune ? *tune : Tuning();
-  g.Pbin = std::max(1, std::min(tn.Pbin > 0 ? tn.Pbin : 8, g.P));
+  g.Pbin = std::max(1, std::min(tn.Pbin > 0 ? tn.Pbin : 4, g.P));
   g.nlayer = (g.P + g.Pbin - 1) / g.Pbin;
-  // pruned LDS FFT: power-of-two grids whose rows fit one workgroup's LDS
+  // pruned LDS FFT: power-of-two grids whose rows and columns fit one
+  // workgroup's LDS, with the image at most half the grid on each axis
   g.fft_lds = 0;
-  g.A2 = g.B2 = g.Wc1 = g.Wc2 = g.rpw = g.lds_row_bytes = g.Erow = 0;
+  g.rpw = g.cpw = g.nct = g.lds_row_bytes = g.lds_col_bytes = 0;
   auto pow2 = [](int n) { return n > 0 && (n & (n - 1)) == 0; };
   const size_t cs = dbl ? 16 : 8;
   const size_t lds_max = tn.lds_bytes > 0 ? (size_t)tn.lds_bytes : size_t(160) << 10;
-  if (tn.fft != 1 && pow2(g.nf1) && pow2(g.nf2) && g.nf1 >= 64 && g.nf1 <= 16384 && g.nf2 >= 256) {
-    int m = 0;
-    while ((1 << m) < g.nf2) ++m;
-    g.A2 = 1 << (m / 2);
-    g.B2 = g.nf2 / g.A2;
-    g.rpw = std::max(1, std::min(4096 / g.nf1, g.nf2));
-    g.Wc1 = std::min(64, std::max(1, 4096 / g.A2));
-    g.Wc2 = std::min(64, std::max(1, 4096 / g.B2));
-    // static LDS of the row kernels (bin walk table) comes on top of the row
-    size_t rb = (size_t)fftc::padded_size(g.rpw * g.nf1) * cs;
-    size_t c1 = (size_t)fftc::padded_size(g.A2 * g.Wc1) * cs, c2 = (size_t)fftc::padded_size(g.B2 * g.Wc2) * cs;
-    g.Erow = 16;
-    if (rb + 2048 <= lds_max && c1 <= lds_max && c2 <= lds_max && g.rpw * g.nf1 / g.Erow <= 1024) {
+  const int max_threads = dbl ? 512 : 1024;
+  if (tn.fft != 1 && pow2(g.nf1) && pow2(g.nf2) && g.nf1 >= 64 && g.nf2 >= 64 && g.nf1 <= 16384 &&
+      g.nf2 <= 16384 && 2 * g.N1 <= g.nf1 && 2 * g.N2 <= g.nf2) {
+    const int rpw = std::max(1, std::min(4096 / g.nf1, g.nf2));
+    const int cpw = std::max(1, std::min(4096 / g.nf2, 16));
+    // the row kernels' static LDS (bin walk table) comes on top of the row
+    const size_t rb = (size_t)(fftc::padded_size(rpw * g.nf1) + fftc::twiddle_entries(g.nf1)) * cs;
+    const size_t cb = (size_t)(fftc::padded_size(cpw * g.nf2) + fftc::twiddle_entries(g.nf2)) * cs;
+    if (rb + 2048 <= lds_max && cb <= lds_max && rpw * g.nf1 / 16 <= max_threads &&
+        cpw * g.nf2 / 16 <= max_threads && rpw * g.nf1 >= 16 * 64 && cpw * g.nf2 >= 16) {
       g.fft_lds = 1;
+      g.rpw = rpw;
+      g.cpw = cpw;
+      g.nct = (g.N1 + 15) / 16;
       g.lds_row_bytes = (int)rb;
+      g.lds_col_bytes = (int)cb;
     }
   }
   // planes per batch from the memory budget: full planes (rocFFT path) or, on
   // the LDS path, the compact plane plus the one or two column-pass buffers
   size_t per_plane = (size_t)g.nf1 * g.nf2 * cs;
-  if (g.fft_lds) per_plane = ((size_t)g.N1 * g.N2 + 2 * (size_t)g.nf2 * g.N1) * cs;
+  if (g.fft_lds) per_plane = (size_t)g.nf2 * g.nct * 16 * cs;
   int Pb = tn.Pb;
   if (Pb <= 0) {
     Pb = (int)std::max<size_t>(1, tn.grid_budget_bytes / per_plane);
@@ -1387,7 +1502,7 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
   // T1 x T2 tiles for the tile-owner spread
   if (g.fft_lds) {
     g.T1 = g.nf1;
-    g.T2 = std::min(tn.T2 > 0 ? tn.T2 : 8, g.nf2);
+    g.T2 = std::min(tn.T2 > 0 ? tn.T2 : 2, g.nf2);
   } else {
     g.T1 = std::min(tn.T1 > 0 ? tn.T1 : (dbl ? 32 : 64), g.nf1);
     g.T2 = std::min(tn.T2 > 0 ? tn.T2 : 32, g.nf2);
@@ -1444,7 +1559,7 @@ struct Bufs {
   double *tabs = nullptr;  // fk1 [nf1/2+1] | fk2 [nf2/2+1] | quadrature nodes | weights
   int *hist = nullptr, *off = nullptr, *bsum = nullptr, *perm = nullptr;
   int2 *binrank = nullptr;
-  void *recs = nullptr, *grid = nullptr, *H = nullptr, *H2 = nullptr;
+  void *recs = nullptr, *grid = nullptr, *H = nullptr;
   double2 *acc = nullptr, *vacc = nullptr;
   const void *tw1 = nullptr, *tw2 = nullptr;
 };
@@ -1500,20 +1615,28 @@ int plan_all(const Problem *pbs, int n, hipStream_t st, const Tuning *tune, int 
   return kOk;
 }
 
-// Twiddle tables tw[t] = e^{+2 pi i t / n} for n = nf1 and n = nf2 (FP64 values,
-// rounded to T), uploaded when the sizes change.
+// Two-level twiddle tables (fftc::TwoLevel: hi | lo) for n = nf1 and n = nf2,
+// FP64 values rounded to T, uploaded when the sizes change.
 template <typename T>
 int twiddles(int dev, const Geom &g, hipStream_t st, const cpx<T> *&tw1, const cpx<T> *&tw2) {
   void *vp;
-  const size_t bytes = sizeof(cpx<T>) * (size_t)(g.nf1 + g.nf2);
+  const int n1 = fftc::twiddle_entries(g.nf1), n2 = fftc::twiddle_entries(g.nf2);
+  const size_t bytes = sizeof(cpx<T>) * (size_t)(n1 + n2);
   CURIG_TRY(ws_get(dev, S_TW, bytes, &vp));
   cpx<T> *t = (cpx<T> *)vp;
   int *key = g_ws[dev].tw_key;
   if (key[0] != g.nf1 || key[1] != g.nf2 || key[2] != (int)sizeof(T)) {
-    std::vector<cpx<T>> h(g.nf1 + g.nf2);
-    for (int i = 0; i < g.nf1; ++i) h[i] = fftc::mk((T)std::cos(2 * M_PI * i / g.nf1), (T)std::sin(2 * M_PI * i / g.nf1));
-    for (int i = 0; i < g.nf2; ++i)
-      h[g.nf1 + i] = fftc::mk((T)std::cos(2 * M_PI * i / g.nf2), (T)std::sin(2 * M_PI * i / g.nf2));
+    std::vector<cpx<T>> h(n1 + n2);
+    auto fill = [](cpx<T> *o, int n) {
+      const int lb = fftc::twiddle_split(n), nhi = n >> lb, nlo = 1 << lb;
+      for (int j = 0; j < nhi; ++j) {
+        const double a = 2 * M_PI * ((double)j * nlo) / n;
+        o[j] = fftc::mk((T)std::cos(a), (T)std::sin(a));
+      }
+      for (int j = 0; j < nlo; ++j) o[nhi + j] = fftc::mk((T)std::cos(2 * M_PI * j / n), (T)std::sin(2 * M_PI * j / n));
+    };
+    fill(h.data(), g.nf1);
+    fill(h.data() + n1, g.nf2);
     CURIG_HIP(hipMemcpyAsync(t, h.data(), bytes, hipMemcpyHostToDevice, st));
     CURIG_HIP(hipStreamSynchronize(st));
     key[0] = g.nf1;
@@ -1521,7 +1644,7 @@ int twiddles(int dev, const Geom &g, hipStream_t st, const cpx<T> *&tw1, const c
     key[2] = (int)sizeof(T);
   }
   tw1 = t;
-  tw2 = t + g.nf1;
+  tw2 = t + n1;
   return kOk;
 }
 
@@ -1540,12 +1663,8 @@ int allow_kernels(const Geom &g) {
   CURIG_TRY(allow_lds(row_ifft_interp_kernel<T, 16, 8>, g.lds_row_bytes));
   CURIG_TRY(allow_lds(row_ifft_interp_kernel<T, 16, 16>, g.lds_row_bytes));
   CURIG_TRY(allow_lds(row_ifft_interp_kernel<T, 16, kMaxKernelWidth>, g.lds_row_bytes));
-  const int l1 = fftc::padded_size(g.A2 * g.Wc1) * (int)sizeof(cpx<T>);
-  const int l2 = fftc::padded_size(g.B2 * g.Wc2) * (int)sizeof(cpx<T>);
-  CURIG_TRY(allow_lds(col_pass1_kernel<T, false>, l1));
-  CURIG_TRY(allow_lds(col_pass1_kernel<T, true>, l1));
-  CURIG_TRY(allow_lds(col_pass2_kernel<T, false>, l2));
-  CURIG_TRY(allow_lds(col_pass2_kernel<T, true>, l2));
+  CURIG_TRY(allow_lds(col_fwd_kernel<T, 16>, g.lds_col_bytes));
+  CURIG_TRY(allow_lds(col_inv_kernel<T, 16>, g.lds_col_bytes));
   return kOk;
 }
 
@@ -1588,8 +1707,8 @@ int reserve(const std::vector<ChanPlan> &cps, const Problem *pbs, bool to_image,
     nrow = std::max(nrow, (size_t)std::max(pbs[i].nrow, 1));
     nvalid = std::max(nvalid, (size_t)std::max<long>(cps[i].nvalid, 1));
     const size_t plane = (size_t)g.nf1 * g.nf2, pix = (size_t)g.N1 * g.N2;
-    grid = std::max(grid, (g.fft_lds ? pix : plane) * g.Pb);
-    h = std::max(h, (size_t)g.nf2 * g.N1 * g.Pb);
+    if (!g.fft_lds) grid = std::max(grid, plane * g.Pb);
+    if (g.fft_lds) h = std::max(h, (size_t)g.nf2 * g.nct * 16 * g.Pb);
     npix = std::max(npix, pix);
   }
   void *vp;
@@ -1616,10 +1735,6 @@ int reserve(const std::vector<ChanPlan> &cps, const Problem *pbs, bool to_image,
   if (g0.fft_lds) {
     CURIG_TRY(ws_get(dev, S_H, sizeof(cpx<T>) * h, &vp));
     b.H = vp;
-    if (!to_image) {
-      CURIG_TRY(ws_get(dev, S_H2, sizeof(cpx<T>) * h, &vp));
-      b.H2 = vp;
-    }
     const cpx<T> *t1, *t2;
     CURIG_TRY(twiddles<T>(dev, g0, st, t1, t2));
     b.tw1 = t1;
@@ -1718,53 +1833,46 @@ void launch_spread_row(const Rec<T> *recs, const int *off, cpx<T> *H, const cpx<
       recs, off, H, tw1, g, p0);
 }
 
-// ms2dirty planes of one batch: spread + row FFT -> H, column passes -> compact Fc [nb][N2][N1]
+// ms2dirty, one batch on the LDS path: spread + row FFT -> H (column-tiled)
 template <typename T>
-void lds_fft_forward(const Rec<T> *recs, const int *off, const Geom &g, int p0, int nb, cpx<T> *Fc, cpx<T> *H,
-                     const cpx<T> *tw1, const cpx<T> *tw2, hipStream_t st, Timer &tm, int &mark_fft) {
-  const int sign = +1;
+void lds_spread_rows(const Rec<T> *recs, const int *off, const Geom &g, int p0, int nb, cpx<T> *H,
+                     const cpx<T> *tw1, hipStream_t st) {
   if (g.kw <= 8)
     launch_spread_row<T, 8>(recs, off, H, tw1, g, p0, nb, st);
   else if (g.kw <= 16)
     launch_spread_row<T, 16>(recs, off, H, tw1, g, p0, nb, st);
   else
     launch_spread_row<T, kMaxKernelWidth>(recs, off, H, tw1, g, p0, nb, st);
-  mark_fft = tm.mark();
-  const int ncb1 = (g.N1 + g.Wc1 - 1) / g.Wc1, ncb2 = (g.N1 + g.Wc2 - 1) / g.Wc2;
-  const int l1 = fftc::padded_size(g.A2 * g.Wc1) * (int)sizeof(cpx<T>);
-  const int l2 = fftc::padded_size(g.B2 * g.Wc2) * (int)sizeof(cpx<T>);
-  col_pass1_kernel<T, false><<<dim3(g.B2 * ncb1, nb), g.A2 * g.Wc1 / 16, l1, st>>>(
-      H, H, tw2, g.nf2, g.N1, g.N2, g.A2, g.B2, g.Wc1, sign);
-  col_pass2_kernel<T, true><<<dim3(g.A2 * ncb2, nb), g.B2 * g.Wc2 / 16, l2, st>>>(
-      H, Fc, tw2, g.nf2, g.N1, g.N2, g.A2, g.B2, g.Wc2, sign);
 }
 
-// dirty2ms planes of one batch, column passes: compact Fc [nb][N2][N1] (in the
-// grid buffer) -> H2 [nb][nf2][N1]
+inline int col_groups(const Geom &g) { return (g.N1 + g.cpw - 1) / g.cpw; }
+
+// ms2dirty, one batch on the LDS path: column FFT + w-accumulation into accT
 template <typename T>
-void lds_fft_backward(const Geom &g, int nb, cpx<T> *G, cpx<T> *H, cpx<T> *H2, const cpx<T> *tw2, hipStream_t st) {
-  const int sign = -1;
-  const int ncb1 = (g.N1 + g.Wc1 - 1) / g.Wc1, ncb2 = (g.N1 + g.Wc2 - 1) / g.Wc2;
-  const int l1 = fftc::padded_size(g.A2 * g.Wc1) * (int)sizeof(cpx<T>);
-  const int l2 = fftc::padded_size(g.B2 * g.Wc2) * (int)sizeof(cpx<T>);
-  col_pass1_kernel<T, true><<<dim3(g.B2 * ncb1, nb), g.A2 * g.Wc1 / 16, l1, st>>>(
-      G, H, tw2, g.nf2, g.N1, g.N2, g.A2, g.B2, g.Wc1, sign);
-  col_pass2_kernel<T, false><<<dim3(g.A2 * ncb2, nb), g.B2 * g.Wc2 / 16, l2, st>>>(
-      H, H2, tw2, g.nf2, g.N1, g.N2, g.A2, g.B2, g.Wc2, sign);
+void lds_cols_forward(const Geom &g, int p0, int nb, const cpx<T> *H, double2 *accT, const cpx<T> *tw2, int first,
+                      hipStream_t st) {
+  col_fwd_kernel<T, 16><<<col_groups(g), g.cpw * g.nf2 / 16, g.lds_col_bytes, st>>>(H, accT, tw2, g, p0, nb, first);
 }
 
-// dirty2ms planes of one batch, row pass fused with the interpolation
+// dirty2ms, one batch on the LDS path: w-fill + inverse column FFT -> H (column-tiled)
 template <typename T>
-void lds_row_interp(const Rec<T> *recs, const int *off, const cpx<T> *H2, double2 *vacc, const cpx<T> *tw1,
+void lds_cols_backward(const Geom &g, int p0, int nb, const cpx<T> *gT, cpx<T> *H, const cpx<T> *tw2,
+                       hipStream_t st) {
+  col_inv_kernel<T, 16><<<col_groups(g), g.cpw * g.nf2 / 16, g.lds_col_bytes, st>>>(gT, H, tw2, g, p0, nb);
+}
+
+// dirty2ms, one batch on the LDS path: inverse row FFT fused with the interpolation
+template <typename T>
+void lds_row_interp(const Rec<T> *recs, const int *off, const cpx<T> *H, double2 *vacc, const cpx<T> *tw1,
                     const Geom &g, int p0, int nb, hipStream_t st) {
   const dim3 grd(g.nf2 / g.rpw, nb);
   const int nt = g.rpw * g.nf1 / 16;
   if (g.kw <= 8)
-    row_ifft_interp_kernel<T, 16, 8><<<grd, nt, g.lds_row_bytes, st>>>(H2, recs, off, vacc, tw1, g, p0, -1);
+    row_ifft_interp_kernel<T, 16, 8><<<grd, nt, g.lds_row_bytes, st>>>(H, recs, off, vacc, tw1, g, p0, -1);
   else if (g.kw <= 16)
-    row_ifft_interp_kernel<T, 16, 16><<<grd, nt, g.lds_row_bytes, st>>>(H2, recs, off, vacc, tw1, g, p0, -1);
+    row_ifft_interp_kernel<T, 16, 16><<<grd, nt, g.lds_row_bytes, st>>>(H, recs, off, vacc, tw1, g, p0, -1);
   else
-    row_ifft_interp_kernel<T, 16, kMaxKernelWidth><<<grd, nt, g.lds_row_bytes, st>>>(H2, recs, off, vacc, tw1, g,
+    row_ifft_interp_kernel<T, 16, kMaxKernelWidth><<<grd, nt, g.lds_row_bytes, st>>>(H, recs, off, vacc, tw1, g,
                                                                                       p0, -1);
 }
 
@@ -1804,24 +1912,31 @@ int enqueue_ms2dirty(const Problem &pb, const ChanPlan &cp, const Bufs &b, hipSt
   double t_spread = 0, t_fft = 0, t_wacc = 0;
   for (int bi = 0; bi < g.nbatch; ++bi) {
     const int p0 = bi * g.Pb, nb = std::min(g.Pb, g.P - p0);
-    int a = tm.mark(), bm;
+    const int a = tm.mark();
     if (g.fft_lds) {
-      // the fused kernel's time counts as spread (t_grid); the column passes as FFT
-      lds_fft_forward<T>(recs, b.off, g, p0, nb, (cpx<T> *)grid, (cpx<T> *)b.H, (const cpx<T> *)b.tw1,
-                         (const cpx<T> *)b.tw2, st, tm, bm);
-    } else {
-      spread_dispatch<T>(recs, b.off, grid, g, p0, nb, st);
-      bm = tm.mark();
-      CURIG_TRY(batch_fft<T>(b.dev, g, nb, +1, grid, st));
+      // spread + row FFT counts as spread (t_grid); the column FFT fused with
+      // the w-accumulation as FFT (t_fft)
+      lds_spread_rows<T>(recs, b.off, g, p0, nb, (cpx<T> *)b.H, (const cpx<T> *)b.tw1, st);
+      CURIG_HIP(hipGetLastError());
+      const int bm = tm.mark();
+      lds_cols_forward<T>(g, p0, nb, (const cpx<T> *)b.H, b.acc, (const cpx<T> *)b.tw2, bi == 0 ? 1 : 0, st);
+      CURIG_HIP(hipGetLastError());
+      const int c = tm.mark();
+      if (tm.on) {
+        CURIG_HIP(hipEventSynchronize(tm.ev[c]));
+        t_spread += tm.ms(a, bm);
+        t_fft += tm.ms(bm, c);
+      }
+      continue;
     }
+    spread_dispatch<T>(recs, b.off, grid, g, p0, nb, st);
+    const int bm = tm.mark();
+    CURIG_TRY(batch_fft<T>(b.dev, g, nb, +1, grid, st));
     CURIG_HIP(hipGetLastError());
-    int c = tm.mark();
-    if (g.fft_lds)
-      wacc_compact_kernel<T><<<nblocks(npix), 256, 0, st>>>((const cpx<T> *)grid, b.acc, g, p0, nb, bi == 0 ? 1 : 0);
-    else
-      wacc_kernel<T><<<nblocks(npix), 256, 0, st>>>(grid, b.acc, g, p0, nb, bi == 0 ? 1 : 0);
+    const int c = tm.mark();
+    wacc_kernel<T><<<nblocks(npix), 256, 0, st>>>(grid, b.acc, g, p0, nb, bi == 0 ? 1 : 0);
     CURIG_HIP(hipGetLastError());
-    int d = tm.mark();
+    const int d = tm.mark();
     if (tm.on) {
       CURIG_HIP(hipEventSynchronize(tm.ev[d]));
       t_spread += tm.ms(a, bm);
@@ -1831,9 +1946,11 @@ int enqueue_ms2dirty(const Problem &pb, const ChanPlan &cp, const Bufs &b, hipSt
   }
   int e2 = tm.mark();
   if (pb.dbl)
-    epilogue_kernel<double><<<nblocks(npix), 256, 0, st>>>(b.acc, pb.dirty, g, fk1, fk2, q, pb.accumulate ? 1 : 0);
+    epilogue_kernel<double><<<nblocks(npix), 256, 0, st>>>(b.acc, pb.dirty, g, fk1, fk2, q, pb.accumulate ? 1 : 0,
+                                                           g.fft_lds);
   else
-    epilogue_kernel<float><<<nblocks(npix), 256, 0, st>>>(b.acc, pb.dirty, g, fk1, fk2, q, pb.accumulate ? 1 : 0);
+    epilogue_kernel<float><<<nblocks(npix), 256, 0, st>>>(b.acc, pb.dirty, g, fk1, fk2, q, pb.accumulate ? 1 : 0,
+                                                          g.fft_lds);
   CURIG_HIP(hipGetLastError());
   int e3 = tm.mark();
   if (sts) {
@@ -1871,38 +1988,49 @@ int enqueue_dirty2ms(const Problem &pb, const ChanPlan &cp, const Bufs &b, hipSt
   v2_t<T> *grid = (v2_t<T> *)b.grid;
   double2 *gimg = b.acc;
   const double *fk1 = b.tabs, *fk2 = b.tabs + (g.nf1 / 2 + 1), *q = fk2 + (g.nf2 / 2 + 1);
-  if (pb.dbl)
+  if (g.fft_lds) {
+    if (pb.dbl)
+      prologue_cols_kernel<double, T><<<nblocks(npix), 256, 0, st>>>(pb.dirty, (cpx<T> *)gimg, g, fk1, fk2, q);
+    else
+      prologue_cols_kernel<float, T><<<nblocks(npix), 256, 0, st>>>(pb.dirty, (cpx<T> *)gimg, g, fk1, fk2, q);
+  } else if (pb.dbl) {
     prologue_kernel<double><<<nblocks(npix), 256, 0, st>>>(pb.dirty, gimg, g, fk1, fk2, q);
-  else
+  } else {
     prologue_kernel<float><<<nblocks(npix), 256, 0, st>>>(pb.dirty, gimg, g, fk1, fk2, q);
+  }
   fill_zero_c128_kernel<<<nblocks(cp.nvalid), 256, 0, st>>>(b.vacc, cp.nvalid);
   CURIG_HIP(hipGetLastError());
   int e1 = tm.mark();
   double t_interp = 0, t_fft = 0, t_fill = 0;
   for (int bi = 0; bi < g.nbatch; ++bi) {
     const int p0 = bi * g.Pb, nb = std::min(g.Pb, g.P - p0);
-    int a = tm.mark();
-    if (g.fft_lds)
-      wfill_compact_kernel<T><<<nblocks(npix), 256, 0, st>>>(gimg, (cpx<T> *)grid, g, p0, nb);
-    else
-      wfill_kernel<T><<<nblocks((long)plane), 256, 0, st>>>(gimg, grid, g, p0, nb);
-    CURIG_HIP(hipGetLastError());
-    int bm = tm.mark();
-    if (g.fft_lds)
-      lds_fft_backward<T>(g, nb, (cpx<T> *)grid, (cpx<T> *)b.H, (cpx<T> *)b.H2, (const cpx<T> *)b.tw2, st);
-    else
-      CURIG_TRY(batch_fft<T>(b.dev, g, nb, -1, grid, st));
-    CURIG_HIP(hipGetLastError());
-    int c = tm.mark();
-    // the fused row iFFT + interpolation counts as interpolation (t_grid)
+    const int a = tm.mark();
     if (g.fft_lds) {
+      // w-fill + column iFFT counts as FFT (t_fft); the row iFFT fused with
+      // the interpolation as interpolation (t_grid)
+      lds_cols_backward<T>(g, p0, nb, (const cpx<T> *)gimg, (cpx<T> *)b.H, (const cpx<T> *)b.tw2, st);
+      CURIG_HIP(hipGetLastError());
+      const int c = tm.mark();
       if (cp.nvalid > 0)
-        lds_row_interp<T>(recs, b.off, (const cpx<T> *)b.H2, b.vacc, (const cpx<T> *)b.tw1, g, p0, nb, st);
-    } else {
-      interp_dispatch<T>(recs, b.off, grid, b.vacc, g, cp.nvalid, p0, nb, st);
+        lds_row_interp<T>(recs, b.off, (const cpx<T> *)b.H, b.vacc, (const cpx<T> *)b.tw1, g, p0, nb, st);
+      CURIG_HIP(hipGetLastError());
+      const int d = tm.mark();
+      if (tm.on) {
+        CURIG_HIP(hipEventSynchronize(tm.ev[d]));
+        t_fft += tm.ms(a, c);
+        t_interp += tm.ms(c, d);
+      }
+      continue;
     }
+    wfill_kernel<T><<<nblocks((long)plane), 256, 0, st>>>(gimg, grid, g, p0, nb);
     CURIG_HIP(hipGetLastError());
-    int d = tm.mark();
+    const int bm = tm.mark();
+    CURIG_TRY(batch_fft<T>(b.dev, g, nb, -1, grid, st));
+    CURIG_HIP(hipGetLastError());
+    const int c = tm.mark();
+    interp_dispatch<T>(recs, b.off, grid, b.vacc, g, cp.nvalid, p0, nb, st);
+    CURIG_HIP(hipGetLastError());
+    const int d = tm.mark();
     if (tm.on) {
       CURIG_HIP(hipEventSynchronize(tm.ev[d]));
       t_fill += tm.ms(a, bm);

@@ -89,3 +89,22 @@ def test_radix8_stages(lib, n, count, sign):
     np.testing.assert_allclose(got, want, atol=1e-11 * np.abs(x).sum() / count)
     got32 = run(lib, x.reshape(-1), n, count, n, 1, 0, sign, 0, maxr=8).reshape(count, n)
     assert np.sqrt(np.mean(np.abs(got32 - want) ** 2)) < 3e-7 * np.sqrt(np.mean(np.abs(want) ** 2))
+
+
+@pytest.mark.parametrize("n", [64, 256, 1024, 4096, 8192, 16384])
+@pytest.mark.parametrize("sign", [1, -1])
+def test_two_level_twiddles(lib, n, sign):
+    """The kernels' LDS twiddles: products of two sqrt(n)-entry tables."""
+    lib.fftc_check_run2.argtypes = [ctypes.c_int] * 7 + [np.ctypeslib.ndpointer(np.float64, flags="C"),
+                                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.fftc_check_run2.restype = ctypes.c_int
+    rng = np.random.default_rng(n + 1)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    want = np.fft.ifft(x) * n if sign > 0 else np.fft.fft(x)
+    for dbl, tol in ((1, 1e-12), (0, 3e-7)):
+        for maxr in (8, 16):
+            data = np.ascontiguousarray(np.stack([x.real, x.imag], -1).reshape(-1))
+            assert lib.fftc_check_run2(n, 1, n, 1, 0, sign, dbl, data, n, 16, maxr, 1) == 0
+            got = data.reshape(-1, 2) @ np.array([1, 1j])
+            err = np.sqrt(np.mean(np.abs(got - want) ** 2)) / np.sqrt(np.mean(np.abs(want) ** 2))
+            assert err < tol, (dbl, maxr, err)

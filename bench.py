@@ -147,20 +147,37 @@ class Runner:
         return total.cpu().numpy()
 
 
-def spread_bytes(plans, nrow, precision, nx, ny):
-    """Algorithmic bytes of the gridding kernel over one ms2dirty (all channels).
+def kernel_bytes(plans, nrow, precision, nx, ny):
+    """Algorithmic HBM bytes of the plane-loop kernels over one ms2dirty and one
+    dirty2ms (all channels), each counted once per launch as the kernel's
+    minimum traffic: every input element read once, every output written once.
 
-    Each visibility is read once (24 B of uvw per row, s_c + s_w of value and
-    weight) and each output cell is written once.  The fused spread + row-FFT
-    kernel (power-of-two grids) writes the row-transformed planes pruned to the
-    image columns, P x nf2 x N1 cells (N1 = ny); the tile spread kernel writes
-    the full planes, P x nf2 x nf1 cells."""
+    s_c = bytes of a complex plane value (8 FP32, 16 FP64); s_w = weight bytes;
+    R rows (24 B of uvw each), M visibilities, per channel P planes, planes of
+    nf2 x N1 row-transformed cells (N1 = ny image columns), N = nx ny pixels,
+    B batches.
+      gridding (spread + row FFT):    24 R + (s_c + s_w) M + s_c P nf2 N1
+      column FFT + w-accumulation:    s_c P nf2 N1 + 16 N (2 B - 1)
+      w-fill + inverse column FFT:    s_c N B + s_c P nf2 N1
+      inverse row FFT + interpolation: s_c P nf2 N1 + (32 + 32) M
+    (the last: one 32-byte sorted record and one FP64 accumulator read and
+    written per visibility).  The tile-spread path (rocFFT planes) writes full
+    planes: 24 R + (s_c + s_w) M + s_c P nf1 nf2."""
     sc, sw = (8, 4) if precision == "fp32" else (16, 8)
-    total = 0
+    npix = nx * ny
+    out = {"grid": 0, "col_fwd": 0, "col_inv": 0, "row_interp": 0, "launches": 0}
     for s in plans:
-        cols = ny if s["fft_lds"] else s["nf1"]
-        total += 24 * nrow + (sc + sw) * s["nvis"] + sc * s["nf3"] * s["nf2"] * cols
-    return total
+        M, P = s["nvis"], s["nf3"]
+        if s["fft_lds"]:
+            plane = sc * P * s["nf2"] * ny
+            out["grid"] += 24 * nrow + (sc + sw) * M + plane
+            out["col_fwd"] += plane + 16 * npix * (2 * s["nbatch"] - 1)
+            out["col_inv"] += sc * npix * s["nbatch"] + plane
+            out["row_interp"] += plane + 64 * M
+        else:
+            out["grid"] += 24 * nrow + (sc + sw) * M + sc * P * s["nf1"] * s["nf2"]
+        out["launches"] += s["nbatch"]
+    return out
 
 
 def timed_steps(run, steps, warmup):
@@ -456,6 +473,10 @@ def main():
     ap.add_argument("--no-api", action="store_true", help="skip the host-pointer (PCIe-inclusive) timing")
     ap.add_argument("--form", default="precision", choices=["precision", "batched", "diff"])
     ap.add_argument("--lr", type=float, default=0.02, help="Adam step of the diff form")
+    ap.add_argument("--traffic", type=float, default=None,
+                    help="HBM bytes per launch of the dominant kernel from a counter run (reported as given)")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run one 1 GiB device copy first (counter calibration of FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build has no multi-GPU "
@@ -470,6 +491,14 @@ def main():
     M = int(cfg["vis"].shape[0] * cfg["vis"].shape[1])
     log(f"[bench] {args.config}: {cfg['vis'].shape[0]} rows x {len(cfg['freqs'])} channels = {M} vis, "
         f"{cfg['nx']}^2, fov {cfg['fov']}, eps {cfg['eps']}, {args.precision}")
+    if args.calib_copy:
+        import torch
+
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     run = Runner(cfg)
     # one channel per launch sequence (no graph capture), so the per-stage HIP
     # events of the timed steps split the gridding kernel from the FFT passes
@@ -478,11 +507,9 @@ def main():
     value = M / t_step / 1e6
     m2d = calls["m2d"][-1]
     log(f"[bench] {t_step * 1e3:.1f} ms/step -> {value:.3f} Mvis/s; last ms2dirty stats {m2d}")
-    # gridding-kernel roofline, from the per-stage HIP events recorded on the
-    # launch stream during the timed steps
-    t_spread = sum(s["t_grid"] for s in calls["m2d"]) / len(calls["m2d"]) * 1e-3
+    # per-kernel achieved bandwidth, from the per-stage HIP events the library
+    # records on the launch stream during the timed steps
     stats_by_ch = []
-    # per-channel plane counts: recompute from the host plan with the data's w range
     from curig_amd import _lib
 
     w = cfg["uvw"][:, 2]
@@ -490,10 +517,21 @@ def main():
         stats_by_ch.append(dict(_lib.plan_info(cfg["vis"].shape[0], cfg["nx"], cfg["ny"], cfg["fov"],
                                                float(cfg["freqs"][ch]), cfg["eps"], 2.0, -1, float(w.min()),
                                                float(w.max()), run.prec), nvis=cfg["vis"].shape[0]))
-    B = spread_bytes(stats_by_ch, cfg["vis"].shape[0], args.precision, cfg["nx"], cfg["ny"])
-    launches = sum(s["nbatch"] for s in stats_by_ch)
-    kname = "spread_row_fft_kernel" if stats_by_ch[0]["fft_lds"] else "spread_tile_kernel"
-    achieved = B / t_spread / 1e9
+    kb = kernel_bytes(stats_by_ch, cfg["vis"].shape[0], args.precision, cfg["nx"], cfg["ny"])
+    launches = kb["launches"]
+    mean = lambda key, d: sum(c[key] for c in calls[d]) / len(calls[d]) * 1e-3
+    lds = bool(stats_by_ch[0]["fft_lds"])
+    kernels = {"spread_row_fft_kernel" if lds else "spread_tile_kernel": (kb["grid"], mean("t_grid", "m2d"))}
+    if lds:
+        kernels["col_fwd_kernel"] = (kb["col_fwd"], mean("t_fft", "m2d"))
+        kernels["col_inv_kernel"] = (kb["col_inv"], mean("t_fft", "d2m"))
+        kernels["row_ifft_interp_kernel"] = (kb["row_interp"], mean("t_grid", "d2m"))
+    per_kernel = {k: {"bytes_per_launch": round(b / launches), "avg_launch_ms": round(t / launches * 1e3, 3),
+                      "achieved_GBps": round(b / t / 1e9, 1), "frac": round(b / t / 1e9 / HBM_PEAK_GBS, 4)}
+                  for k, (b, t) in kernels.items() if t > 0}
+    kname = max(kernels, key=lambda k: kernels[k][1])
+    B, t_dom = kernels[kname]
+    achieved = B / t_dom / 1e9
     result = {
         "metric": "Mvis/s ms2dirty+dirty2ms, FP32-mixed vs FP64 at 1e-5 err; % HBM roofline",
         "value": round(value, 4),
@@ -515,9 +553,9 @@ def main():
                    "kw": stats_by_ch[0]["kw"]},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "launches_per_ms2dirty": launches,
-                     "bytes_per_launch": round(B / launches), "avg_launch_ms": round(t_spread / launches * 1e3, 4),
-                     "kernel_ms_per_ms2dirty": round(t_spread * 1e3, 2)},
+                     "traffic": args.traffic, "launches_per_call": launches,
+                     "bytes_per_launch": round(B / launches), "avg_launch_ms": round(t_dom / launches * 1e3, 3),
+                     "kernels": per_kernel},
         "stages_ms": {"ms2dirty": {k: round(v, 2) for k, v in m2d.items() if k.startswith("t_")},
                       "dirty2ms": {k: round(v, 2) for k, v in calls["d2m"][-1].items() if k.startswith("t_")}},
     }

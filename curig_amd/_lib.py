@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcurafft.so")
+# CURIG_LIBRARY selects another build of the library (same exports)
+LIB_PATH = os.environ.get("CURIG_LIBRARY") or os.path.join(HERE, "libcurafft.so")
 
 c_int, c_double, c_long, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_long, ctypes.c_void_p
 _f64 = np.ctypeslib.ndpointer(np.double, flags="C")

@@ -35,13 +35,19 @@ def _flags():
     ]
 
 
-def build(verbose=False, jobs=4):
-    os.makedirs(OBJDIR, exist_ok=True)
+def build(verbose=False, jobs=4, defines=(), out=None, objdir=None):
+    """Compile and link the library.  defines: extra -D macros (e.g. the
+    layout constants of curig_internal.h); out / objdir: another library path
+    and object folder (default: the package's libcurafft.so)."""
+    objdir = objdir or OBJDIR
+    out = out or LIB
+    os.makedirs(objdir, exist_ok=True)
     hipcc = _hipcc()
+    dflags = [f"-D{d}" for d in defines]
 
     def compile_one(src):
-        obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
-        cmd = [hipcc] + _flags() + ["-c", os.path.join(CSRC, src), "-o", obj]
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        cmd = [hipcc] + _flags() + dflags + ["-c", os.path.join(CSRC, src), "-o", obj]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
@@ -51,14 +57,14 @@ def build(verbose=False, jobs=4):
 
     with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(compile_one, SOURCES))
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + [
         "-L/opt/rocm/lib", "-lrocfft", "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

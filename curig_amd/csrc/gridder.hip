@@ -1004,12 +1004,12 @@ __device__ __forceinline__ void interp_rows(const Rec<T> &rc, int idx, const cpx
 }
 
 
-// Row-transformed planes live in a column-tiled layout: element (plane k, grid
-// row k2, image column col) at ((k * nct + col / 16) * nf2 + k2) * 16 + col % 16,
-// so a row's 16-column chunks are contiguous (row kernels) and a column's
-// elements are 16 apart (column kernels; the 16 columns of a tile run on one
-// XCD, whose L2 merges their accesses into whole lines).
-constexpr int kTile = 16;
+// The intermediate planes live in a column-tiled layout: element (plane k,
+// grid row k2, image column col) at ((k * nct + col / kTile) * nf2 + k2) * kTile
+// + col % kTile, so a row's kTile-column chunks are contiguous for the row
+// kernels and a column's elements are kTile apart for the column kernels (the
+// kTile columns of a tile run on one XCD, whose L2 serves the shared sectors).
+constexpr int kTile = 1 << kTileLog2;
 
 // Workgroup id -> work item such that each XCD (workgroups are dispatched to
 // the 8 XCDs round robin) takes one contiguous range of items.
@@ -1056,7 +1056,7 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) spread_row_fft_kernel(co
   // pruned write: all LDS reads first, then the stores (16-column chunks);
   // offsets from the row's base in 32 bits (a plane holds < 2^31 elements)
   const int N1 = g.N1, nout = rpw * N1, h1 = N1 / 2;
-  const int lgt = 35 - __clz(g.nf2);  // log2(nf2 * 16)
+  const int lgt = 31 - __clz(g.nf2) + kTileLog2;  // log2(nf2 * kTile)
   cpx<T> *hb = H + ((size_t)pl * g.nct * g.nf2 + row0) * kTile;
   int tid = threadIdx.x;
   __asm__ volatile("" : "+v"(tid));
@@ -1075,12 +1075,13 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) spread_row_fft_kernel(co
     const int i = tid + e * nt;
     if (i < nout) {
       const int r = rpw == 1 ? 0 : i / N1, col = i - r * N1;
-      hb[((col >> 4) << lgt) + (r << 4) + (col & 15)] = o[e];
+      hb[((col >> kTileLog2) << lgt) + (r << kTileLog2) + (col & (kTile - 1))] = o[e];
     }
   }
 }
 
-// dirty2ms: row pass of the column-transformed planes (column-tiled H, each row
+// dirty2ms: row pass of the column-transformed planes (H as written by
+// col_inv_kernel, each row
 // expanded to nf1 cells, zero outside the image columns), inverse row FFT in
 // LDS, then interpolation of every record touching the workgroup's rows of
 // plane p: per record a partial sum, added to its FP64 accumulator (atomics),
@@ -1096,16 +1097,30 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) row_ifft_interp_kernel(c
   __shared__ WalkTable wt;
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
   const int rpw = g.rpw, nf1 = g.nf1, N1 = g.N1, nt = blockDim.x;
-  const int row0 = blockIdx.x * rpw, pl = blockIdx.y, p = p0 + pl;
-  const int lg = 31 - __clz(nf1), lgt = 35 - __clz(g.nf2);
-  const cpx<T> *hb = H + ((size_t)pl * g.nct * g.nf2 + row0) * kTile;
+  // rows in XCD-contiguous order: the 16 rows sharing a column line run together
+  const int row0 = xcd_group(blockIdx.x, gridDim.x) * rpw, pl = blockIdx.y, p = p0 + pl;
+  const int lg = 31 - __clz(nf1), lg2 = 31 - __clz(g.nf2);
   cpx<T> v[E];
+  if (kD2mColMajor) {
+    const cpx<T> *hb = H + (size_t)pl * g.nct * kTile * g.nf2 + row0;
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int i = threadIdx.x + e * nt;  // < rpw * nf1 = E * nt
-    const int r = i >> lg, k1 = i & (nf1 - 1);
-    const int col = image_index(k1, N1, nf1);
-    v[e] = col >= 0 ? hb[((col >> 4) << lgt) + (r << 4) + (col & 15)] : fftc::mk((T)0, (T)0);
+    for (int e = 0; e < E; ++e) {
+      const int i = threadIdx.x + e * nt;  // < rpw * nf1 = E * nt
+      const int r = i >> lg, k1 = i & (nf1 - 1);
+      const int col = image_index(k1, N1, nf1);
+      v[e] = col >= 0 ? hb[(col << lg2) + r] : fftc::mk((T)0, (T)0);
+    }
+  } else {
+    const cpx<T> *hb = H + ((size_t)pl * g.nct * g.nf2 + row0) * kTile;
+    const int lgt = lg2 + kTileLog2;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = threadIdx.x + e * nt;
+      const int r = i >> lg, k1 = i & (nf1 - 1);
+      const int col = image_index(k1, N1, nf1);
+      v[e] = col >= 0 ? hb[((col >> kTileLog2) << lgt) + (r << kTileLog2) + (col & (kTile - 1))]
+                      : fftc::mk((T)0, (T)0);
+    }
   }
   const auto tv = lds_twiddles<T>(lds + fftc::padded_size(rpw * nf1), tw, nf1);
   if (threadIdx.x < 64) walk_table_build(wt, off, g, p, row0, rpw);
@@ -1152,7 +1167,7 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) col_fwd_kernel(const cpx
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
   const int cpw = g.cpw, nf2 = g.nf2, N1 = g.N1, N2 = g.N2, nt = blockDim.x;
   const int col0 = xcd_group(blockIdx.x, gridDim.x) * cpw;
-  const int lg = 31 - __clz(nf2), lgt = lg + 4;
+  const int lg = 31 - __clz(nf2), lgt = lg + kTileLog2;
   const int nout = cpw * N2;
   double ar[8], ai[8], zp[8];
 #pragma unroll
@@ -1176,7 +1191,8 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) col_fwd_kernel(const cpx
     for (int e = 0; e < E; ++e) {
       const int i = tid + e * nt;
       const int ci = i >> lg, k2 = i & (nf2 - 1), col = col0 + ci;
-      v[e] = col < N1 ? hk[((col >> 4) << lgt) + (k2 << 4) + (col & 15)] : fftc::mk((T)0, (T)0);
+      v[e] = col < N1 ? hk[((col >> kTileLog2) << lgt) + (k2 << kTileLog2) + (col & (kTile - 1))]
+                      : fftc::mk((T)0, (T)0);
     }
     __syncthreads();  // the previous plane's LDS reads are done
 #pragma unroll
@@ -1225,7 +1241,7 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) col_fwd_kernel(const cpx
 // workgroup owns cpw grid columns and, for each of the batch's nb planes, places
 // gT[col][row] e^{-i z' (p - P/2)} (FP64 phase argument) at the image rows of its columns
 // (zeros elsewhere), runs the inverse FFT over nf2 and writes the full columns
-// to the column-tiled layout H.
+// to H (column-tiled, or column-major when kD2mColMajor).
 template <typename T, int E>
 __global__ void __launch_bounds__(MaxThreads<T>::value) col_inv_kernel(const cpx<T> *__restrict__ gT,
                                                                       cpx<T> *__restrict__ H,
@@ -1235,7 +1251,7 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) col_inv_kernel(const cpx
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
   const int cpw = g.cpw, nf2 = g.nf2, N1 = g.N1, N2 = g.N2, nt = blockDim.x;
   const int col0 = xcd_group(blockIdx.x, gridDim.x) * cpw;
-  const int lg = 31 - __clz(nf2), lgt = lg + 4;
+  const int lg = 31 - __clz(nf2);
   const int nout = cpw * N2, nzero = cpw * (nf2 - N2), h2 = N2 / 2;
   cpx<T> gv[8];
   double zp[8];
@@ -1282,12 +1298,18 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) col_inv_kernel(const cpx
     cpx<T> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = lds[fftc::pad(tid2 + e * nt)];
+    // column-major output: each column's nf2 cells contiguous
     cpx<T> *hk = H + (size_t)k * g.nct * nf2 * kTile;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = tid2 + e * nt;
       const int ci = i >> lg, k2 = i & (nf2 - 1), col = col0 + ci;
-      if (col < N1) hk[((col >> 4) << lgt) + (k2 << 4) + (col & 15)] = v[e];
+      if (col < N1) {
+        if (kD2mColMajor)
+          hk[(col << lg) + k2] = v[e];
+        else
+          hk[((col >> kTileLog2) << (lg + kTileLog2)) + (k2 << kTileLog2) + (col & (kTile - 1))] = v[e];
+      }
     }
   }
 }
@@ -1482,7 +1504,7 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
       g.fft_lds = 1;
       g.rpw = rpw;
       g.cpw = cpw;
-      g.nct = (g.N1 + 15) / 16;
+      g.nct = (g.N1 + kTile - 1) / kTile;
       g.lds_row_bytes = (int)rb;
       g.lds_col_bytes = (int)cb;
     }
@@ -1490,7 +1512,7 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
   // planes per batch from the memory budget: full planes (rocFFT path) or, on
   // the LDS path, the compact plane plus the one or two column-pass buffers
   size_t per_plane = (size_t)g.nf1 * g.nf2 * cs;
-  if (g.fft_lds) per_plane = (size_t)g.nf2 * g.nct * 16 * cs;
+  if (g.fft_lds) per_plane = (size_t)g.nf2 * g.nct * kTile * cs;
   int Pb = tn.Pb;
   if (Pb <= 0) {
     Pb = (int)std::max<size_t>(1, tn.grid_budget_bytes / per_plane);
@@ -1708,7 +1730,7 @@ int reserve(const std::vector<ChanPlan> &cps, const Problem *pbs, bool to_image,
     nvalid = std::max(nvalid, (size_t)std::max<long>(cps[i].nvalid, 1));
     const size_t plane = (size_t)g.nf1 * g.nf2, pix = (size_t)g.N1 * g.N2;
     if (!g.fft_lds) grid = std::max(grid, plane * g.Pb);
-    if (g.fft_lds) h = std::max(h, (size_t)g.nf2 * g.nct * 16 * g.Pb);
+    if (g.fft_lds) h = std::max(h, (size_t)g.nf2 * g.nct * kTile * g.Pb);
     npix = std::max(npix, pix);
   }
   void *vp;

@@ -1080,10 +1080,9 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) spread_row_fft_kernel(co
   }
 }
 
-// dirty2ms: row pass of the column-transformed planes (H as written by
-// col_inv_kernel, each row
-// expanded to nf1 cells, zero outside the image columns), inverse row FFT in
-// LDS, then interpolation of every record touching the workgroup's rows of
+// dirty2ms: row pass of the column-transformed planes (H as col_inv_kernel
+// writes it; each row expanded to nf1 cells, zero outside the image columns),
+// inverse row FFT in LDS, then interpolation of every record touching the workgroup's rows of
 // plane p: per record a partial sum, added to its FP64 accumulator (atomics),
 // so the full grid planes are never written.
 template <typename T, int E, int KWM>

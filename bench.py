@@ -219,7 +219,12 @@ def cpu_baseline(cfg, target_s=12.0):
         if dt > 0.5 or nr >= len(vis):
             break
         nr = min(len(vis), nr * 4)
-    nr = int(min(len(vis), max(nr, nr * target_s / max(dt, 1e-3))))
+    scale = target_s / max(dt, 1e-3)
+    nr_new = int(min(len(vis), max(nr, nr * scale)))
+    # all rows in use: make up the rest of the sample with more pixels
+    npix = int(min(nx * ny, max(npix, npix * scale * nr / nr_new)))
+    nr = nr_new
+    pix = rng.choice(nx * ny, npix, replace=False)
     t0 = time.perf_counter()
     O.dft_ms2dirty_pixels(uvw[:nr], vis[:nr], None, nx, ny, fov, f0, pix, -1, threads)
     dt = time.perf_counter() - t0
@@ -459,6 +464,20 @@ def form_diff(args, log):
     }
 
 
+def traffic_for(kernel, args):
+    """HBM bytes per launch of `kernel` measured by the FETCH_SIZE / WRITE_SIZE
+    counter runs recorded in profiles/ (calibrated against a 1 GiB copy), for
+    the default workload; --traffic overrides; None when not measured."""
+    if args.traffic is not None:
+        return args.traffic
+    path = os.path.join(ROOT, "profiles", f"{args.config}_{args.precision}_traffic.json")
+    if args.eps is not None or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    return rec.get("bytes_per_launch", {}).get(kernel)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -471,6 +490,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-err", action="store_true")
     ap.add_argument("--no-api", action="store_true", help="skip the host-pointer (PCIe-inclusive) timing")
+    ap.add_argument("--no-eps4", action="store_true", help="skip the epsilon 1e-4 run of the same workload")
     ap.add_argument("--form", default="precision", choices=["precision", "batched", "diff"])
     ap.add_argument("--lr", type=float, default=0.02, help="Adam step of the diff form")
     ap.add_argument("--traffic", type=float, default=None,
@@ -553,22 +573,34 @@ def main():
                    "kw": stats_by_ch[0]["kw"]},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": args.traffic, "launches_per_call": launches,
+                     "traffic": traffic_for(kname, args), "launches_per_call": launches,
                      "bytes_per_launch": round(B / launches), "avg_launch_ms": round(t_dom / launches * 1e3, 3),
                      "kernels": per_kernel},
         "stages_ms": {"ms2dirty": {k: round(v, 2) for k, v in m2d.items() if k.startswith("t_")},
                       "dirty2ms": {k: round(v, 2) for k, v in calls["d2m"][-1].items() if k.startswith("t_")}},
     }
+    rng = np.random.default_rng(7)
+    pix = rng.choice(cfg["nx"] * cfg["ny"], 10000, replace=False)
+    img32 = None
     if not args.no_err:
-        rng = np.random.default_rng(7)
-        pix = rng.choice(cfg["nx"] * cfg["ny"], 10000, replace=False)
         t0 = time.perf_counter()
         truth = run.explicit_pixels(pix)
-        img = run.d_img.cpu().numpy().reshape(-1)[pix]
-        err = float(np.linalg.norm(img - truth) / np.linalg.norm(truth))
+        img32 = run.d_img.cpu().numpy().reshape(-1)[pix].astype(np.complex128)
+        err = float(np.linalg.norm(img32 - truth) / np.linalg.norm(truth))
         result["error_vs_fp64_direct_sum"] = {"pixels": 10000, "rel_l2": err,
                                              "seconds": round(time.perf_counter() - t0, 1)}
         log(f"[bench] error vs FP64 direct sum on 10k pixels: {err:.3e}")
+    if args.precision == "fp32" and not args.no_eps4 and cfg["eps"] != 1e-4:
+        # the same workload at epsilon 1e-4 (kernel width 5)
+        run.cfg = dict(cfg, eps=1e-4)
+        t4, _ = timed_steps(run, 1, 1)
+        e4 = {"epsilon": 1e-4, "value": round(M / t4 / 1e6, 4), "ms_per_step": round(t4 * 1e3, 1)}
+        if not args.no_err:
+            img4 = run.d_img.cpu().numpy().reshape(-1)[pix].astype(np.complex128)
+            e4["error_vs_fp64_direct_sum"] = float(np.linalg.norm(img4 - truth) / np.linalg.norm(truth))
+        result["epsilon_1e-4"] = e4
+        log(f"[bench] eps 1e-4: {e4}")
+        run.cfg = cfg
     del run
     if not args.no_api:
         t_api = api_step_seconds(cfg)
@@ -584,7 +616,14 @@ def main():
         v64 = M / t64 / 1e6
         result["fp64_path"] = {"value": round(v64, 4), "ms_per_step": round(t64 * 1e3, 1),
                                "fp32_speedup": round(value / v64, 3)}
-        log(f"[bench] fp64 path: {v64:.3f} Mvis/s, speedup {value / v64:.2f}x")
+        if not args.no_err:
+            # the FP64 path's own error (its inputs are the unrounded FP64 values)
+            truth64 = run64.explicit_pixels(pix)
+            img64 = run64.d_img.cpu().numpy().reshape(-1)[pix]
+            result["fp64_path"]["error_vs_fp64_direct_sum"] = float(np.linalg.norm(img64 - truth64) /
+                                                                    np.linalg.norm(truth64))
+            result["fp64_path"]["fp32_vs_fp64_image"] = float(np.linalg.norm(img32 - img64) / np.linalg.norm(img64))
+        log(f"[bench] fp64 path: {result['fp64_path']}")
         del run64
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(cfg)

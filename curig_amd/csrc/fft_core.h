@@ -43,6 +43,25 @@ template <typename T> FC_HD cpx<T> mulw(cpx<T> a, T c, T s) { return mk(a.x * c 
 FC_HD int pad(int i) { return i + (i >> 4); }
 FC_HD int padded_size(int n) { return n + (n >> 4) + 1; }
 
+// Padded positions of a butterfly's R elements e0 + r * st as p0 + r * pst when
+// that holds exactly: st a multiple of 16 (each step adds st / 16 pad slots), or
+// all R elements inside one 16-element block (no pad slot between them).  With
+// compile-time sizes both tests fold, and a butterfly needs one address.
+FC_HD bool run_offsets(int e0, int st, int R, int &p0, int &pst) {
+  p0 = pad(e0);
+  if ((st & 15) == 0) {
+    pst = st + (st >> 4);
+    return true;
+  }
+  if ((e0 & 15) + (R - 1) * st < 16) {
+    pst = st;
+    return true;
+  }
+  pst = 0;
+  return false;
+}
+
+
 
 // cos / sin of 2 pi t / 16 as constant expressions (always folded at compile time)
 FC_HD constexpr double cos16(int t) {
@@ -60,6 +79,25 @@ FC_HD int bitrev(int x, int bits) {
   return r;
 }
 
+// a e^{sign 2 pi i t / 16} for a constant t (after unrolling): the multiples of
+// pi/4 are formed without general products (IEEE rules keep the compiler from
+// dropping y * 0, so 1, +-i and (1 +- i)/sqrt2 are written out).
+template <typename T> FC_HD cpx<T> rot16(cpx<T> a, int t, int sign) {
+  const int q = ((sign > 0 ? t : 16 - t) & 15);
+  const T h = (T)0.70710678118654752440;
+  switch (q) {
+    case 0: return a;
+    case 4: return mk(-a.y, a.x);
+    case 8: return mk(-a.x, -a.y);
+    case 12: return mk(a.y, -a.x);
+    case 2: return mk((a.x - a.y) * h, (a.x + a.y) * h);
+    case 6: return mk(-(a.x + a.y) * h, (a.x - a.y) * h);
+    case 10: return mk((a.y - a.x) * h, -(a.x + a.y) * h);
+    case 14: return mk((a.x + a.y) * h, (a.y - a.x) * h);
+    default: return mulw(a, (T)cos16(q), (T)sin16(q));
+  }
+}
+
 // In-register DFT of size R (2, 4, 8, 16): X[q] = sum_r v[r] e^{sign 2 pi i q r / R}
 template <typename T, int R> FC_HD void dft_small(cpx<T> *v, int sign) {
   constexpr int L = R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4;
@@ -73,8 +111,7 @@ template <typename T, int R> FC_HD void dft_small(cpx<T> *v, int sign) {
 #pragma unroll
       for (int j = 0; j < len / 2; ++j) {
         const int t = j * (16 / len);
-        const T c = (T)cos16(t), s = (T)(sign * sin16(t));
-        cpx<T> u = a[st + j], w = mulw(a[st + j + len / 2], c, s);
+        cpx<T> u = a[st + j], w = rot16(a[st + j + len / 2], t, sign);
         a[st + j] = add(u, w);
         a[st + j + len / 2] = sub(u, w);
       }
@@ -126,8 +163,14 @@ FC_HD void stage_load(const Buf buf, cpx<T> *v, const Layout &L, int Ns, int sig
     const int j = L.seq_fast ? beta / L.count : beta % nbf;
     const int base = seq * L.seq_stride;
     cpx<T> *u = v + b * R;
+    int p0, pst;
+    if (run_offsets(base + j * L.elem_stride, nbf * L.elem_stride, R, p0, pst)) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) u[r] = buf[pad(base + (j + r * nbf) * L.elem_stride)];
+      for (int r = 0; r < R; ++r) u[r] = buf[p0 + r * pst];
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) u[r] = buf[pad(base + (j + r * nbf) * L.elem_stride)];
+    }
     const int k = j & (Ns - 1);
     if (Ns > 1) {
       cpx<T> w[R];
@@ -156,8 +199,14 @@ FC_HD void stage_store(Buf buf, const cpx<T> *v, const Layout &L, int Ns, int ti
     const int k = j & (Ns - 1);
     const int d = (j - k) * R + k;
     const cpx<T> *u = v + b * R;
+    int p0, pst;
+    if (run_offsets(base + d * L.elem_stride, Ns * L.elem_stride, R, p0, pst)) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) buf[pad(base + (d + r * Ns) * L.elem_stride)] = u[r];
+      for (int r = 0; r < R; ++r) buf[p0 + r * pst] = u[r];
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[pad(base + (d + r * Ns) * L.elem_stride)] = u[r];
+    }
 #if defined(__HIP_DEVICE_COMPILE__)
     __builtin_amdgcn_sched_barrier(0);
 #endif

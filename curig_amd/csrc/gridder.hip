@@ -1035,7 +1035,9 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) spread_row_fft_kernel(co
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
   T *ldsf = reinterpret_cast<T *>(smem_raw);
   const int rpw = g.rpw, nf1 = g.nf1, nt = blockDim.x;
-  const int row0 = blockIdx.x * rpw, pl = blockIdx.y, p = p0 + pl;
+  // rows in XCD-contiguous order: the rows sharing a tile's lines are written
+  // through one L2, which merges their chunks into whole lines
+  const int row0 = xcd_group(blockIdx.x, gridDim.x) * rpw, pl = blockIdx.y, p = p0 + pl;
   const int ntot = fftc::padded_size(rpw * nf1);
   const auto tv = lds_twiddles<T>(lds + ntot, tw, nf1);
   if (threadIdx.x < 64) walk_table_build(wt, off, g, p, row0, rpw);

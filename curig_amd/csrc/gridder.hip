@@ -1045,6 +1045,17 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) spread_row_fft_kernel(co
   __syncthreads();
   const T beta = (T)g.beta, cc = (T)g.c, hw = (T)(0.5 * g.kw);
   const int total = wt.total;
+  const int N1 = g.N1, nout = rpw * N1, h1 = N1 / 2;
+  const int lgt = 31 - __clz(g.nf2) + kTileLog2;  // log2(nf2 * kTile)
+  cpx<T> *hb = H + ((size_t)pl * g.nct * g.nf2 + row0) * kTile;
+  if (total == 0) {
+    // no visibility reaches these rows of this plane: the transform is zero
+    for (int i = threadIdx.x; i < nout; i += nt) {
+      const int r = rpw == 1 ? 0 : i / N1, col = i - r * N1;
+      hb[((col >> kTileLog2) << lgt) + (r << kTileLog2) + (col & (kTile - 1))] = fftc::mk((T)0, (T)0);
+    }
+    return;
+  }
   for (int t0 = threadIdx.x; t0 < total; t0 += 2 * nt) {
     const int t1 = t0 + nt;
     const Rec<T> ra = recs[walk_record(wt, t0)];
@@ -1055,11 +1066,8 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) spread_row_fft_kernel(co
   }
   __syncthreads();
   lds_fft_rows<T, E, 16>(lds, 31 - __clz(nf1), rpw, +1, tv);
-  // pruned write: all LDS reads first, then the stores (16-column chunks);
+  // pruned write: all LDS reads first, then the stores (kTile-column chunks);
   // offsets from the row's base in 32 bits (a plane holds < 2^31 elements)
-  const int N1 = g.N1, nout = rpw * N1, h1 = N1 / 2;
-  const int lgt = 31 - __clz(g.nf2) + kTileLog2;  // log2(nf2 * kTile)
-  cpx<T> *hb = H + ((size_t)pl * g.nct * g.nf2 + row0) * kTile;
   int tid = threadIdx.x;
   __asm__ volatile("" : "+v"(tid));
   cpx<T> o[E];
@@ -1129,6 +1137,7 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) row_ifft_interp_kernel(c
   for (int e = 0; e < E; ++e) lds[fftc::pad(threadIdx.x + e * nt)] = v[e];
   __syncthreads();
   const int total = wt.total;
+  if (total == 0) return;  // no visibility reaches these rows of this plane
   // first record of this thread, loaded while the FFT runs
   int idx0 = 0;
   Rec<T> r0;

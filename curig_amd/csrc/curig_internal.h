@@ -92,19 +92,14 @@ struct Geom {
   int lds_col_bytes;  // dynamic LDS of the column-pass workgroup
 };
 
-// Column-tile width of the intermediate planes (LDS path): 2^kTileLog2 image
-// columns per contiguous chunk.  4 columns (32 bytes FP32) measured fastest at
-// 8192^2 on MI355X (2: row writes too narrow; 8 and 16: column reads too
-// sparse).  dirty2ms's planes are tiled the same way, or column-major when
-// kD2mColMajor (slower: strided row reads).
+// Column-tile width of the intermediate planes (LDS path, both directions):
+// 2^kTileLog2 image columns per contiguous chunk.  4 columns (32 bytes FP32)
+// measured fastest at 8192^2 on MI355X (2: row writes too narrow; 8 and 16:
+// column reads too sparse; column-major planes for dirty2ms: strided row reads).
 #ifndef CURIG_TILE_LOG2
 #define CURIG_TILE_LOG2 2
 #endif
-#ifndef CURIG_D2M_COLMAJOR
-#define CURIG_D2M_COLMAJOR 0
-#endif
 constexpr int kTileLog2 = CURIG_TILE_LOG2;
-constexpr bool kD2mColMajor = CURIG_D2M_COLMAJOR != 0;
 
 // Tunables (autotuned per problem class; defaults are safe for any size)
 struct Tuning {

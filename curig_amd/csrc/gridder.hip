@@ -1110,26 +1110,15 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) row_ifft_interp_kernel(c
   const int row0 = xcd_group(blockIdx.x, gridDim.x) * rpw, pl = blockIdx.y, p = p0 + pl;
   const int lg = 31 - __clz(nf1), lg2 = 31 - __clz(g.nf2);
   cpx<T> v[E];
-  if (kD2mColMajor) {
-    const cpx<T> *hb = H + (size_t)pl * g.nct * kTile * g.nf2 + row0;
+  const cpx<T> *hb = H + ((size_t)pl * g.nct * g.nf2 + row0) * kTile;
+  const int lgt = lg2 + kTileLog2;
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int i = threadIdx.x + e * nt;  // < rpw * nf1 = E * nt
-      const int r = i >> lg, k1 = i & (nf1 - 1);
-      const int col = image_index(k1, N1, nf1);
-      v[e] = col >= 0 ? hb[(col << lg2) + r] : fftc::mk((T)0, (T)0);
-    }
-  } else {
-    const cpx<T> *hb = H + ((size_t)pl * g.nct * g.nf2 + row0) * kTile;
-    const int lgt = lg2 + kTileLog2;
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int i = threadIdx.x + e * nt;
-      const int r = i >> lg, k1 = i & (nf1 - 1);
-      const int col = image_index(k1, N1, nf1);
-      v[e] = col >= 0 ? hb[((col >> kTileLog2) << lgt) + (r << kTileLog2) + (col & (kTile - 1))]
-                      : fftc::mk((T)0, (T)0);
-    }
+  for (int e = 0; e < E; ++e) {
+    const int i = threadIdx.x + e * nt;  // < rpw * nf1 = E * nt
+    const int r = i >> lg, k1 = i & (nf1 - 1);
+    const int col = image_index(k1, N1, nf1);
+    v[e] = col >= 0 ? hb[((col >> kTileLog2) << lgt) + (r << kTileLog2) + (col & (kTile - 1))]
+                    : fftc::mk((T)0, (T)0);
   }
   const auto tv = lds_twiddles<T>(lds + fftc::padded_size(rpw * nf1), tw, nf1);
   if (threadIdx.x < 64) walk_table_build(wt, off, g, p, row0, rpw);
@@ -1251,7 +1240,7 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) col_fwd_kernel(const cpx
 // workgroup owns cpw grid columns and, for each of the batch's nb planes, places
 // gT[col][row] e^{-i z' (p - P/2)} (FP64 phase argument) at the image rows of its columns
 // (zeros elsewhere), runs the inverse FFT over nf2 and writes the full columns
-// to H (column-tiled, or column-major when kD2mColMajor).
+// to the column-tiled H.
 template <typename T, int E>
 __global__ void __launch_bounds__(MaxThreads<T>::value) col_inv_kernel(const cpx<T> *__restrict__ gT,
                                                                       cpx<T> *__restrict__ H,
@@ -1308,18 +1297,12 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) col_inv_kernel(const cpx
     cpx<T> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = lds[fftc::pad(tid2 + e * nt)];
-    // column-major output: each column's nf2 cells contiguous
     cpx<T> *hk = H + (size_t)k * g.nct * nf2 * kTile;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = tid2 + e * nt;
       const int ci = i >> lg, k2 = i & (nf2 - 1), col = col0 + ci;
-      if (col < N1) {
-        if (kD2mColMajor)
-          hk[(col << lg) + k2] = v[e];
-        else
-          hk[((col >> kTileLog2) << (lg + kTileLog2)) + (k2 << kTileLog2) + (col & (kTile - 1))] = v[e];
-      }
+      if (col < N1) hk[((col >> kTileLog2) << (lg + kTileLog2)) + (k2 << kTileLog2) + (col & (kTile - 1))] = v[e];
     }
   }
 }

@@ -90,6 +90,8 @@ struct Geom {
   int nct;            // column tiles of a plane's row-transformed layout (ceil(N1 / kTile))
   int lds_row_bytes;  // dynamic LDS of the row-pass workgroup
   int lds_col_bytes;  // dynamic LDS of the column-pass workgroup
+  int rhalf;          // 1: rows run as two half-length workgroups (too long for two per CU)
+  int lds_rowh_bytes; // dynamic LDS of a half-row workgroup
 };
 
 // Column-tile width of the intermediate planes (LDS path, both directions):

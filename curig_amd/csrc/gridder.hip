@@ -883,7 +883,8 @@ __device__ __forceinline__ void lds_fft(cpx<T> *buf, const fftc::Layout &L, int 
 // Stockham stages of `count` contiguous length-N sequences (buffer index
 // pad(s * N + k)) with N a compile-time constant: every stage's radix, stride
 // and twiddle step fold to constants, and the element indexing to shifts.
-template <typename T, int E, int MAXR, int N, int NS, typename Tw>
+// (the twiddle table has TWM * N entries)
+template <typename T, int E, int MAXR, int N, int NS, int TWM = 1, typename Tw>
 __device__ __forceinline__ void fixed_stages(cpx<T> *buf, int count, int sign, const Tw &tw) {
   if constexpr (NS < N) {
     constexpr int R = (NS * MAXR <= N) ? MAXR : N / NS;
@@ -892,11 +893,22 @@ __device__ __forceinline__ void fixed_stages(cpx<T> *buf, int count, int sign, c
     int tid = threadIdx.x;
     __asm__ volatile("" : "+v"(tid));
     cpx<T> v[E];
-    fftc::stage_load<T, E, R>(buf, v, L, NS, sign, tw, N, tid, nt);
+    fftc::stage_load<T, E, R>(buf, v, L, NS, sign, tw, N * TWM, tid, nt);
     __syncthreads();
     fftc::stage_store<T, E, R>(buf, v, L, NS, tid, nt);
     __syncthreads();
-    fixed_stages<T, E, MAXR, N, NS * R>(buf, count, sign, tw);
+    fixed_stages<T, E, MAXR, N, NS * R, TWM>(buf, count, sign, tw);
+  }
+}
+
+// FFT of one sequence of length m = 2^lgm (m = nf / 2 of a split row or
+// column) with the twiddle table of the full length nf = 2m.
+template <typename T, int E, int MAXR, typename Tw>
+__device__ __forceinline__ void lds_fft_half(cpx<T> *buf, int lgm, int sign, const Tw &tw) {
+  switch (lgm) {
+    case 11: fixed_stages<T, E, MAXR, 2048, 1, 2>(buf, 1, sign, tw); break;
+    case 12: fixed_stages<T, E, MAXR, 4096, 1, 2>(buf, 1, sign, tw); break;
+    default: fixed_stages<T, E, MAXR, 8192, 1, 2>(buf, 1, sign, tw); break;
   }
 }
 
@@ -1140,6 +1152,90 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) row_ifft_interp_kernel(c
   for (int t = threadIdx.x + nt; t < total; t += nt) {
     const int idx = walk_record(wt, t);
     interp_rows<T, KWM>(recs[idx], idx, lds, g, p, row0, vacc, beta, cc, hw);
+  }
+}
+
+// Interpolation of one record from one half of a split row: the row's cells
+// are x[n] = Y0[n mod m] + w^(sign n) Y1[n mod m] (w = e^{2 pi i / nf1}, m = nf1 / 2),
+// and half h holds Yh; its share of the record's sum is added to the record's
+// FP64 accumulator.
+template <typename T, int KWM, typename Tw>
+__device__ __forceinline__ void interp_row_half(const Rec<T> &rc, int idx, const cpx<T> *lds, const Tw &tv,
+                                                const Geom &g, int p, int row0, int h, int sign,
+                                                double2 *__restrict__ vacc, T beta, T cc, T hw) {
+  const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2, mm = (nf1 >> 1) - 1;
+  int k = p - rc.iw;
+  if (k < 0) k += g.P;
+  if (k >= kw) return;
+  int jj = row0 - rc.iy;
+  if (jj < 0) jj += nf2;
+  if (jj >= kw) return;
+  T rr = 0, ri = 0;
+#pragma unroll
+  for (int i = 0; i < KWM; ++i) {
+    if (i < kw) {
+      int cx = rc.ix + i;
+      if (cx >= nf1) cx -= nf1;
+      const T kx = es_eval(rc.dx + (T)i, beta, cc, hw);
+      cpx<T> f = lds[fftc::pad(cx & mm)];
+      if (h) {
+        const cpx<T> w = tv[cx];
+        f = fftc::mulw(f, w.x, (T)sign * w.y);
+      }
+      rr += f.x * kx;
+      ri += f.y * kx;
+    }
+  }
+  const T kyz = es_eval(rc.dy + (T)jj, beta, cc, hw) * es_eval(rc.dw + (T)k, beta, cc, hw);
+  atomicAdd(&vacc[idx].x, (double)(rr * kyz));
+  atomicAdd(&vacc[idx].y, (double)(ri * kyz));
+}
+
+// dirty2ms row pass for rows too long for two workgroups per CU: each row is
+// split into its even- and odd-index halves of length m = nf1 / 2 (radix-2
+// decimation in frequency of the inverse transform), one workgroup per half:
+// Yh = FFT_m over X[2k + h], so the pair of a row loads disjoint halves of H,
+// runs two m-point FFTs and adds its two shares to each record's accumulator.
+// Two such workgroups fit one CU, so one's memory phases overlap the other's FFT.
+template <typename T, int E, int KWM>
+__global__ void __launch_bounds__(MaxThreads<T>::value / 2) row_ifft_interp_half_kernel(
+    const cpx<T> *__restrict__ H, const Rec<T> *__restrict__ recs, const int *__restrict__ off,
+    double2 *__restrict__ vacc, const cpx<T> *__restrict__ tw, Geom g, int p0, int sign) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ WalkTable wt;
+  cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
+  const int nf1 = g.nf1, m = nf1 >> 1, N1 = g.N1, nt = blockDim.x;
+  // the two halves of a row are consecutive items: same XCD, adjacent in time
+  const int item = xcd_group(blockIdx.x, gridDim.x);
+  const int row0 = item >> 1, h = item & 1, pl = blockIdx.y, p = p0 + pl;
+  const int lgm = 30 - __clz(nf1), lgt = 31 - __clz(g.nf2) + kTileLog2;
+  const cpx<T> *hb = H + ((size_t)pl * g.nct * g.nf2 + row0) * kTile;
+  cpx<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = threadIdx.x + e * nt;  // < m = E * nt
+    const int col = image_index(2 * i + h, N1, nf1);
+    v[e] = col >= 0 ? hb[((col >> kTileLog2) << lgt) + (col & (kTile - 1))] : fftc::mk((T)0, (T)0);
+  }
+  const auto tv = lds_twiddles<T>(lds + fftc::padded_size(m), tw, nf1);
+  if (threadIdx.x < 64) walk_table_build(wt, off, g, p, row0, 1);
+#pragma unroll
+  for (int e = 0; e < E; ++e) lds[fftc::pad(threadIdx.x + e * nt)] = v[e];
+  __syncthreads();
+  const int total = wt.total;
+  if (total == 0) return;
+  int idx0 = 0;
+  Rec<T> r0;
+  if (threadIdx.x < total) {
+    idx0 = walk_record(wt, threadIdx.x);
+    r0 = recs[idx0];
+  }
+  lds_fft_half<T, E, 16>(lds, lgm, sign, tv);
+  const T beta = (T)g.beta, cc = (T)g.c, hw = (T)(0.5 * g.kw);
+  if (threadIdx.x < total) interp_row_half<T, KWM>(r0, idx0, lds, tv, g, p, row0, h, sign, vacc, beta, cc, hw);
+  for (int t = threadIdx.x + nt; t < total; t += nt) {
+    const int idx = walk_record(wt, t);
+    interp_row_half<T, KWM>(recs[idx], idx, lds, tv, g, p, row0, h, sign, vacc, beta, cc, hw);
   }
 }
 
@@ -1480,7 +1576,7 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
   // pruned LDS FFT: power-of-two grids whose rows and columns fit one
   // workgroup's LDS, with the image at most half the grid on each axis
   g.fft_lds = 0;
-  g.rpw = g.cpw = g.nct = g.lds_row_bytes = g.lds_col_bytes = 0;
+  g.rpw = g.cpw = g.nct = g.lds_row_bytes = g.lds_col_bytes = g.rhalf = g.lds_rowh_bytes = 0;
   auto pow2 = [](int n) { return n > 0 && (n & (n - 1)) == 0; };
   const size_t cs = dbl ? 16 : 8;
   const size_t lds_max = tn.lds_bytes > 0 ? (size_t)tn.lds_bytes : size_t(160) << 10;
@@ -1500,6 +1596,10 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
       g.nct = (g.N1 + kTile - 1) / kTile;
       g.lds_row_bytes = (int)rb;
       g.lds_col_bytes = (int)cb;
+      // rows too long for two workgroups per CU run as two half-length halves
+      const size_t hb = (size_t)(fftc::padded_size(g.nf1 / 2) + fftc::twiddle_entries(g.nf1)) * cs;
+      g.rhalf = (rpw == 1 && rb + 2048 > lds_max / 2 && g.nf1 >= 4096) ? 1 : 0;
+      g.lds_rowh_bytes = (int)hb;
     }
   }
   // planes per batch from the memory budget: full planes (rocFFT path) or, on
@@ -1678,6 +1778,9 @@ int allow_kernels(const Geom &g) {
   CURIG_TRY(allow_lds(row_ifft_interp_kernel<T, 16, 8>, g.lds_row_bytes));
   CURIG_TRY(allow_lds(row_ifft_interp_kernel<T, 16, 16>, g.lds_row_bytes));
   CURIG_TRY(allow_lds(row_ifft_interp_kernel<T, 16, kMaxKernelWidth>, g.lds_row_bytes));
+  CURIG_TRY(allow_lds(row_ifft_interp_half_kernel<T, 16, 8>, g.lds_rowh_bytes));
+  CURIG_TRY(allow_lds(row_ifft_interp_half_kernel<T, 16, 16>, g.lds_rowh_bytes));
+  CURIG_TRY(allow_lds(row_ifft_interp_half_kernel<T, 16, kMaxKernelWidth>, g.lds_rowh_bytes));
   CURIG_TRY(allow_lds(col_fwd_kernel<T, 16>, g.lds_col_bytes));
   CURIG_TRY(allow_lds(col_inv_kernel<T, 16>, g.lds_col_bytes));
   return kOk;
@@ -1880,6 +1983,18 @@ void lds_cols_backward(const Geom &g, int p0, int nb, const cpx<T> *gT, cpx<T> *
 template <typename T>
 void lds_row_interp(const Rec<T> *recs, const int *off, const cpx<T> *H, double2 *vacc, const cpx<T> *tw1,
                     const Geom &g, int p0, int nb, hipStream_t st) {
+  if (g.rhalf) {
+    const dim3 grd(2 * g.nf2, nb);
+    const int nt = g.nf1 / 32;
+    if (g.kw <= 8)
+      row_ifft_interp_half_kernel<T, 16, 8><<<grd, nt, g.lds_rowh_bytes, st>>>(H, recs, off, vacc, tw1, g, p0, -1);
+    else if (g.kw <= 16)
+      row_ifft_interp_half_kernel<T, 16, 16><<<grd, nt, g.lds_rowh_bytes, st>>>(H, recs, off, vacc, tw1, g, p0, -1);
+    else
+      row_ifft_interp_half_kernel<T, 16, kMaxKernelWidth><<<grd, nt, g.lds_rowh_bytes, st>>>(H, recs, off, vacc,
+                                                                                            tw1, g, p0, -1);
+    return;
+  }
   const dim3 grd(g.nf2 / g.rpw, nb);
   const int nt = g.rpw * g.nf1 / 16;
   if (g.kw <= 8)

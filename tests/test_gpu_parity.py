@@ -404,6 +404,46 @@ def test_lds_fft_path_equals_rocfft_path(precision):
     assert O.l2error(v_roc, v_lds) < tol
 
 
+@pytest.mark.parametrize("precision,n", [("fp64", 4096), ("fp32", 8192)])
+def test_split_rows_equal_rocfft_path(precision, n):
+    """Grids whose rows exceed half a CU's LDS (16384-point FP32, 8192-point
+    FP64) run each row as two half-length workgroups; the result equals the
+    rocFFT path and, on a few visibilities, the FP64 direct sum."""
+    lib = _lib()
+    say(f"[split rows] {precision} {n}x{n}")
+    rng = np.random.default_rng(7)
+    f0, nrow = 1e9, 20000
+    # u, v over the whole grid (every row and column of it is touched), small w
+    dx = 1.0 * np.pi / 180 / n
+    uvw = np.empty((nrow, 3))
+    uvw[:, :2] = (rng.random((nrow, 2)) - 0.5) * 0.98 / dx * C / f0
+    uvw[:, 2] = rng.uniform(-1, 1, nrow) * 30.0
+    wgt = rng.uniform(0.5, 1.0, nrow)
+    img = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    fn_v = gpu_dirty2ms if precision == "fp64" else (lambda *a: gpu_f32(False, *a))
+    eps = 1e-6
+    info = _lib_plan(lib, len(uvw), n, n, 1.0, f0, eps, uvw, precision)
+    assert info["fft_lds"] == 1
+    v_lds = fn_v(uvw, img, wgt, n, n, 1.0, f0, eps)
+    assert lib.curig_set_fft_path(1) == 0
+    try:
+        v_roc = fn_v(uvw, img, wgt, n, n, 1.0, f0, eps)
+    finally:
+        lib.curig_set_fft_path(0)
+    tol = 1e-11 if precision == "fp64" else 2e-6
+    assert O.l2error(v_roc, v_lds) < tol
+    sel = rng.choice(len(uvw), 64, replace=False)
+    want = O.dft_dirty2ms(uvw[sel], img, None if wgt is None else wgt[sel], n, n, 1.0, f0, -1)
+    assert O.l2error(want, np.asarray(v_lds)[sel]) < 3 * eps + (3e-6 if precision == "fp32" else 0)
+
+
+def _lib_plan(lib, nrow, nx, ny, fov, f0, eps, uvw, precision):
+    from curig_amd import _lib as L
+
+    return L.plan_info(nrow, nx, ny, fov, f0, eps, 2.0, -1, float(uvw[:, 2].min()), float(uvw[:, 2].max()),
+                       0 if precision == "fp64" else 1)
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_graph_mode_matches_channel_loop(precision):
     """Batched form: the hipGraph-captured multi-channel sequence equals the plain

@@ -1155,6 +1155,107 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) row_ifft_interp_kernel(c
   }
 }
 
+// Spread of one record into one half of a split row (ms2dirty): half h holds
+// z_h[n mod m] = sum over the row's cells n of x[n] w^(sign h n), whose m-point
+// FFT gives the row transform at the indices 2k + h (radix-2 decimation in
+// frequency; w = e^{2 pi i / nf1}, m = nf1 / 2).
+template <typename T, int KWM, typename Tw>
+__device__ __forceinline__ void spread_row_half(const Rec<T> &rc, T *ldsf, const Tw &tv, const Geom &g, int p,
+                                                int row0, int h, int sign, T beta, T cc, T hw) {
+  const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2, mm = (nf1 >> 1) - 1;
+  int k = p - rc.iw;
+  if (k < 0) k += g.P;
+  if (k >= kw) return;
+  int jj = row0 - rc.iy;
+  if (jj < 0) jj += nf2;
+  if (jj >= kw) return;
+  const T a = es_eval(rc.dw + (T)k, beta, cc, hw) * es_eval(rc.dy + (T)jj, beta, cc, hw);
+  const T vre = rc.re * a, vim = rc.im * a;
+#pragma unroll
+  for (int i = 0; i < KWM; ++i) {
+    if (i < kw) {
+      int cx = rc.ix + i;
+      if (cx >= nf1) cx -= nf1;
+      const T kx = es_eval(rc.dx + (T)i, beta, cc, hw);
+      T re = vre * kx, im = vim * kx;
+      if (h) {
+        const cpx<T> w = tv[cx];
+        const T ws = (T)sign * w.y, t = re * w.x - im * ws;
+        im = re * ws + im * w.x;
+        re = t;
+      }
+      const int q = 2 * fftc::pad(cx & mm);
+      atomicAdd(ldsf + q, re);
+      atomicAdd(ldsf + q + 1, im);
+    }
+  }
+}
+
+// ms2dirty row pass for rows too long for two workgroups per CU: one workgroup
+// per half of a row (the even- or odd-index outputs of the row transform, see
+// spread_row_half), each spreading the row's visibilities into an m-point LDS
+// buffer, transforming it and writing its half of the N1 image columns.
+template <typename T, int E, int KWM>
+__global__ void __launch_bounds__(MaxThreads<T>::value / 2) spread_row_fft_half_kernel(
+    const Rec<T> *__restrict__ recs, const int *__restrict__ off, cpx<T> *__restrict__ H,
+    const cpx<T> *__restrict__ tw, Geom g, int p0) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ WalkTable wt;
+  cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
+  T *ldsf = reinterpret_cast<T *>(smem_raw);
+  const int nf1 = g.nf1, m = nf1 >> 1, nt = blockDim.x, sign = +1;
+  const int item = xcd_group(blockIdx.x, gridDim.x);
+  const int row0 = item >> 1, h = item & 1, pl = blockIdx.y, p = p0 + pl;
+  const int ntot = fftc::padded_size(m);
+  const auto tv = lds_twiddles<T>(lds + ntot, tw, nf1);
+  if (threadIdx.x < 64) walk_table_build(wt, off, g, p, row0, 1);
+  for (int i = threadIdx.x; i < ntot; i += nt) lds[i] = fftc::mk((T)0, (T)0);
+  __syncthreads();
+  const int total = wt.total;
+  // this half's image columns: k = grid_index(col) with k = h mod 2
+  const int N1 = g.N1, h1 = N1 / 2, c0 = h ^ (h1 & 1), nout = (N1 - c0 + 1) >> 1;
+  const int lgt = 31 - __clz(g.nf2) + kTileLog2;
+  cpx<T> *hb = H + ((size_t)pl * g.nct * g.nf2 + row0) * kTile;
+  if (total == 0) {
+    for (int i = threadIdx.x; i < nout; i += nt) {
+      const int col = 2 * i + c0;
+      hb[((col >> kTileLog2) << lgt) + (col & (kTile - 1))] = fftc::mk((T)0, (T)0);
+    }
+    return;
+  }
+  const T beta = (T)g.beta, cc = (T)g.c, hw = (T)(0.5 * g.kw);
+  for (int t0 = threadIdx.x; t0 < total; t0 += 2 * nt) {
+    const int t1 = t0 + nt;
+    const Rec<T> ra = recs[walk_record(wt, t0)];
+    Rec<T> rb;
+    if (t1 < total) rb = recs[walk_record(wt, t1)];
+    spread_row_half<T, KWM>(ra, ldsf, tv, g, p, row0, h, sign, beta, cc, hw);
+    if (t1 < total) spread_row_half<T, KWM>(rb, ldsf, tv, g, p, row0, h, sign, beta, cc, hw);
+  }
+  __syncthreads();
+  lds_fft_half<T, E, 16>(lds, 30 - __clz(nf1), sign, tv);
+  int tid = threadIdx.x;
+  __asm__ volatile("" : "+v"(tid));
+  cpx<T> o[E / 2];
+#pragma unroll
+  for (int e = 0; e < E / 2; ++e) {
+    const int i = tid + e * nt;
+    if (i < nout) {
+      const int col = 2 * i + c0;
+      const int k1 = col >= h1 ? col - h1 : nf1 + col - h1;
+      o[e] = lds[fftc::pad(k1 >> 1)];
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < E / 2; ++e) {
+    const int i = tid + e * nt;
+    if (i < nout) {
+      const int col = 2 * i + c0;
+      hb[((col >> kTileLog2) << lgt) + (col & (kTile - 1))] = o[e];
+    }
+  }
+}
+
 // Interpolation of one record from one half of a split row: the row's cells
 // are x[n] = Y0[n mod m] + w^(sign n) Y1[n mod m] (w = e^{2 pi i / nf1}, m = nf1 / 2),
 // and half h holds Yh; its share of the record's sum is added to the record's
@@ -1778,6 +1879,9 @@ int allow_kernels(const Geom &g) {
   CURIG_TRY(allow_lds(row_ifft_interp_kernel<T, 16, 8>, g.lds_row_bytes));
   CURIG_TRY(allow_lds(row_ifft_interp_kernel<T, 16, 16>, g.lds_row_bytes));
   CURIG_TRY(allow_lds(row_ifft_interp_kernel<T, 16, kMaxKernelWidth>, g.lds_row_bytes));
+  CURIG_TRY(allow_lds(spread_row_fft_half_kernel<T, 16, 8>, g.lds_rowh_bytes));
+  CURIG_TRY(allow_lds(spread_row_fft_half_kernel<T, 16, 16>, g.lds_rowh_bytes));
+  CURIG_TRY(allow_lds(spread_row_fft_half_kernel<T, 16, kMaxKernelWidth>, g.lds_rowh_bytes));
   CURIG_TRY(allow_lds(row_ifft_interp_half_kernel<T, 16, 8>, g.lds_rowh_bytes));
   CURIG_TRY(allow_lds(row_ifft_interp_half_kernel<T, 16, 16>, g.lds_rowh_bytes));
   CURIG_TRY(allow_lds(row_ifft_interp_half_kernel<T, 16, kMaxKernelWidth>, g.lds_rowh_bytes));
@@ -1955,6 +2059,18 @@ void launch_spread_row(const Rec<T> *recs, const int *off, cpx<T> *H, const cpx<
 template <typename T>
 void lds_spread_rows(const Rec<T> *recs, const int *off, const Geom &g, int p0, int nb, cpx<T> *H,
                      const cpx<T> *tw1, hipStream_t st) {
+  if (g.rhalf) {
+    const dim3 grd(2 * g.nf2, nb);
+    const int nt = g.nf1 / 32;
+    if (g.kw <= 8)
+      spread_row_fft_half_kernel<T, 16, 8><<<grd, nt, g.lds_rowh_bytes, st>>>(recs, off, H, tw1, g, p0);
+    else if (g.kw <= 16)
+      spread_row_fft_half_kernel<T, 16, 16><<<grd, nt, g.lds_rowh_bytes, st>>>(recs, off, H, tw1, g, p0);
+    else
+      spread_row_fft_half_kernel<T, 16, kMaxKernelWidth><<<grd, nt, g.lds_rowh_bytes, st>>>(recs, off, H, tw1, g,
+                                                                                           p0);
+    return;
+  }
   if (g.kw <= 8)
     launch_spread_row<T, 8>(recs, off, H, tw1, g, p0, nb, st);
   else if (g.kw <= 16)

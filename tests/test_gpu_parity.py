@@ -404,7 +404,7 @@ def test_lds_fft_path_equals_rocfft_path(precision):
     assert O.l2error(v_roc, v_lds) < tol
 
 
-@pytest.mark.parametrize("precision,n", [("fp64", 4096), ("fp32", 8192)])
+@pytest.mark.parametrize("precision,n", [("fp64", 4096), ("fp32", 8192), ("fp32", 8191)])
 def test_split_rows_equal_rocfft_path(precision, n):
     """Grids whose rows exceed half a CU's LDS (16384-point FP32, 8192-point
     FP64) run each row as two half-length workgroups; the result equals the
@@ -420,18 +420,26 @@ def test_split_rows_equal_rocfft_path(precision, n):
     uvw[:, 2] = rng.uniform(-1, 1, nrow) * 30.0
     wgt = rng.uniform(0.5, 1.0, nrow)
     img = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    vis = rng.standard_normal(nrow) + 1j * rng.standard_normal(nrow)
+    fn_d = gpu_ms2dirty if precision == "fp64" else (lambda *a: gpu_f32(True, *a))
     fn_v = gpu_dirty2ms if precision == "fp64" else (lambda *a: gpu_f32(False, *a))
     eps = 1e-6
     info = _lib_plan(lib, len(uvw), n, n, 1.0, f0, eps, uvw, precision)
     assert info["fft_lds"] == 1
+    d_lds = fn_d(uvw, vis, wgt, n, n, 1.0, f0, eps)
     v_lds = fn_v(uvw, img, wgt, n, n, 1.0, f0, eps)
     assert lib.curig_set_fft_path(1) == 0
     try:
+        d_roc = fn_d(uvw, vis, wgt, n, n, 1.0, f0, eps)
         v_roc = fn_v(uvw, img, wgt, n, n, 1.0, f0, eps)
     finally:
         lib.curig_set_fft_path(0)
     tol = 1e-11 if precision == "fp64" else 2e-6
+    assert O.l2error(d_roc, d_lds) < tol
     assert O.l2error(v_roc, v_lds) < tol
+    pix = rng.choice(n * n, 64, replace=False)
+    want_d = O.dft_ms2dirty_pixels(uvw, vis, wgt, n, n, 1.0, f0, pix, -1)
+    assert O.l2error(want_d, np.asarray(d_lds).reshape(-1)[pix]) < 3 * eps + (3e-6 if precision == "fp32" else 0)
     sel = rng.choice(len(uvw), 64, replace=False)
     want = O.dft_dirty2ms(uvw[sel], img, None if wgt is None else wgt[sel], n, n, 1.0, f0, -1)
     assert O.l2error(want, np.asarray(v_lds)[sel]) < 3 * eps + (3e-6 if precision == "fp32" else 0)

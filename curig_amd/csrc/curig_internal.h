@@ -90,8 +90,8 @@ struct Geom {
   int nct;            // column tiles of a plane's row-transformed layout (ceil(N1 / kTile))
   int lds_row_bytes;  // dynamic LDS of the row-pass workgroup
   int lds_col_bytes;  // dynamic LDS of the column-pass workgroup
-  int rhalf;          // 1: rows run as two half-length workgroups (too long for two per CU)
-  int lds_rowh_bytes; // dynamic LDS of a half-row workgroup
+  int rsplit;         // > 1: rows run as rsplit workgroups of nf1 / rsplit (too long for two per CU)
+  int lds_rowp_bytes; // dynamic LDS of a row-part workgroup
 };
 
 // Column-tile width of the intermediate planes (LDS path, both directions):
@@ -102,6 +102,11 @@ struct Geom {
 #define CURIG_TILE_LOG2 2
 #endif
 constexpr int kTileLog2 = CURIG_TILE_LOG2;
+// Parts a long row is split into (2 or 4; see the row-part kernels).
+#ifndef CURIG_ROW_SPLIT
+#define CURIG_ROW_SPLIT 2
+#endif
+constexpr int kRowSplit = CURIG_ROW_SPLIT;
 
 // Tunables (autotuned per problem class; defaults are safe for any size)
 struct Tuning {

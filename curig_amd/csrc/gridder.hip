@@ -901,14 +901,15 @@ __device__ __forceinline__ void fixed_stages(cpx<T> *buf, int count, int sign, c
   }
 }
 
-// FFT of one sequence of length m = 2^lgm (m = nf / 2 of a split row or
-// column) with the twiddle table of the full length nf = 2m.
-template <typename T, int E, int MAXR, typename Tw>
-__device__ __forceinline__ void lds_fft_half(cpx<T> *buf, int lgm, int sign, const Tw &tw) {
+// FFT of one sequence of length m = 2^lgm (m = nf / S of a row split S ways)
+// with the twiddle table of the full length nf = S m.
+template <typename T, int E, int MAXR, int S, typename Tw>
+__device__ __forceinline__ void lds_fft_part(cpx<T> *buf, int lgm, int sign, const Tw &tw) {
   switch (lgm) {
-    case 11: fixed_stages<T, E, MAXR, 2048, 1, 2>(buf, 1, sign, tw); break;
-    case 12: fixed_stages<T, E, MAXR, 4096, 1, 2>(buf, 1, sign, tw); break;
-    default: fixed_stages<T, E, MAXR, 8192, 1, 2>(buf, 1, sign, tw); break;
+    case 10: fixed_stages<T, E, MAXR, 1024, 1, S>(buf, 1, sign, tw); break;
+    case 11: fixed_stages<T, E, MAXR, 2048, 1, S>(buf, 1, sign, tw); break;
+    case 12: fixed_stages<T, E, MAXR, 4096, 1, S>(buf, 1, sign, tw); break;
+    default: fixed_stages<T, E, MAXR, 8192, 1, S>(buf, 1, sign, tw); break;
   }
 }
 
@@ -1155,14 +1156,14 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) row_ifft_interp_kernel(c
   }
 }
 
-// Spread of one record into one half of a split row (ms2dirty): half h holds
-// z_h[n mod m] = sum over the row's cells n of x[n] w^(sign h n), whose m-point
-// FFT gives the row transform at the indices 2k + h (radix-2 decimation in
-// frequency; w = e^{2 pi i / nf1}, m = nf1 / 2).
+// Spread of one record into part h of a row split S ways (ms2dirty): part h
+// holds z_h[n mod m] = sum over the row's cells n of x[n] w^(sign h n), whose
+// m-point FFT gives the row transform at the indices S k + h (radix-S
+// decimation in frequency; w = e^{2 pi i / nf1}, m = nf1 / S).
 template <typename T, int KWM, typename Tw>
-__device__ __forceinline__ void spread_row_half(const Rec<T> &rc, T *ldsf, const Tw &tv, const Geom &g, int p,
-                                                int row0, int h, int sign, T beta, T cc, T hw) {
-  const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2, mm = (nf1 >> 1) - 1;
+__device__ __forceinline__ void spread_row_part(const Rec<T> &rc, T *ldsf, const Tw &tv, const Geom &g, int p,
+                                                int row0, int h, int mm, int sign, T beta, T cc, T hw) {
+  const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2;
   int k = p - rc.iw;
   if (k < 0) k += g.P;
   if (k >= kw) return;
@@ -1179,7 +1180,7 @@ __device__ __forceinline__ void spread_row_half(const Rec<T> &rc, T *ldsf, const
       const T kx = es_eval(rc.dx + (T)i, beta, cc, hw);
       T re = vre * kx, im = vim * kx;
       if (h) {
-        const cpx<T> w = tv[cx];
+        const cpx<T> w = tv[(h * cx) & (nf1 - 1)];
         const T ws = (T)sign * w.y, t = re * w.x - im * ws;
         im = re * ws + im * w.x;
         re = t;
@@ -1192,33 +1193,36 @@ __device__ __forceinline__ void spread_row_half(const Rec<T> &rc, T *ldsf, const
 }
 
 // ms2dirty row pass for rows too long for two workgroups per CU: one workgroup
-// per half of a row (the even- or odd-index outputs of the row transform, see
-// spread_row_half), each spreading the row's visibilities into an m-point LDS
-// buffer, transforming it and writing its half of the N1 image columns.
-template <typename T, int E, int KWM>
-__global__ void __launch_bounds__(MaxThreads<T>::value / 2) spread_row_fft_half_kernel(
+// per part h of a row split S ways (the row transform's outputs S k + h, see
+// spread_row_part), each spreading the row's visibilities into an m-point LDS
+// buffer, transforming it and writing its share of the N1 image columns (those
+// whose grid index is h mod S).  Several such workgroups fit one CU, so one's
+// memory phases overlap another's FFT.
+template <typename T, int E, int KWM, int S>
+__global__ void __launch_bounds__(MaxThreads<T>::value / S) spread_row_fft_part_kernel(
     const Rec<T> *__restrict__ recs, const int *__restrict__ off, cpx<T> *__restrict__ H,
     const cpx<T> *__restrict__ tw, Geom g, int p0) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ WalkTable wt;
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
   T *ldsf = reinterpret_cast<T *>(smem_raw);
-  const int nf1 = g.nf1, m = nf1 >> 1, nt = blockDim.x, sign = +1;
+  const int nf1 = g.nf1, m = nf1 / S, nt = blockDim.x, sign = +1;
+  // the parts of a row are consecutive items: same XCD, adjacent in time
   const int item = xcd_group(blockIdx.x, gridDim.x);
-  const int row0 = item >> 1, h = item & 1, pl = blockIdx.y, p = p0 + pl;
+  const int row0 = item / S, h = item % S, pl = blockIdx.y, p = p0 + pl;
   const int ntot = fftc::padded_size(m);
   const auto tv = lds_twiddles<T>(lds + ntot, tw, nf1);
   if (threadIdx.x < 64) walk_table_build(wt, off, g, p, row0, 1);
   for (int i = threadIdx.x; i < ntot; i += nt) lds[i] = fftc::mk((T)0, (T)0);
   __syncthreads();
   const int total = wt.total;
-  // this half's image columns: k = grid_index(col) with k = h mod 2
-  const int N1 = g.N1, h1 = N1 / 2, c0 = h ^ (h1 & 1), nout = (N1 - c0 + 1) >> 1;
+  // this part's image columns: grid_index(col) = col - N1/2 (mod nf1) = h (mod S)
+  const int N1 = g.N1, h1 = N1 / 2, c0 = (h + h1) % S, nout = (N1 - c0 + S - 1) / S;
   const int lgt = 31 - __clz(g.nf2) + kTileLog2;
   cpx<T> *hb = H + ((size_t)pl * g.nct * g.nf2 + row0) * kTile;
   if (total == 0) {
     for (int i = threadIdx.x; i < nout; i += nt) {
-      const int col = 2 * i + c0;
+      const int col = S * i + c0;
       hb[((col >> kTileLog2) << lgt) + (col & (kTile - 1))] = fftc::mk((T)0, (T)0);
     }
     return;
@@ -1229,42 +1233,43 @@ __global__ void __launch_bounds__(MaxThreads<T>::value / 2) spread_row_fft_half_
     const Rec<T> ra = recs[walk_record(wt, t0)];
     Rec<T> rb;
     if (t1 < total) rb = recs[walk_record(wt, t1)];
-    spread_row_half<T, KWM>(ra, ldsf, tv, g, p, row0, h, sign, beta, cc, hw);
-    if (t1 < total) spread_row_half<T, KWM>(rb, ldsf, tv, g, p, row0, h, sign, beta, cc, hw);
+    spread_row_part<T, KWM>(ra, ldsf, tv, g, p, row0, h, m - 1, sign, beta, cc, hw);
+    if (t1 < total) spread_row_part<T, KWM>(rb, ldsf, tv, g, p, row0, h, m - 1, sign, beta, cc, hw);
   }
   __syncthreads();
-  lds_fft_half<T, E, 16>(lds, 30 - __clz(nf1), sign, tv);
+  lds_fft_part<T, E, 16, S>(lds, 31 - __clz(m), sign, tv);
   int tid = threadIdx.x;
   __asm__ volatile("" : "+v"(tid));
-  cpx<T> o[E / 2];
+  constexpr int NO = E / 2;  // nout <= m / 2 = (E / 2) nt, since N1 <= nf1 / 2
+  cpx<T> o[NO];
 #pragma unroll
-  for (int e = 0; e < E / 2; ++e) {
+  for (int e = 0; e < NO; ++e) {
     const int i = tid + e * nt;
     if (i < nout) {
-      const int col = 2 * i + c0;
+      const int col = S * i + c0;
       const int k1 = col >= h1 ? col - h1 : nf1 + col - h1;
-      o[e] = lds[fftc::pad(k1 >> 1)];
+      o[e] = lds[fftc::pad(k1 / S)];
     }
   }
 #pragma unroll
-  for (int e = 0; e < E / 2; ++e) {
+  for (int e = 0; e < NO; ++e) {
     const int i = tid + e * nt;
     if (i < nout) {
-      const int col = 2 * i + c0;
+      const int col = S * i + c0;
       hb[((col >> kTileLog2) << lgt) + (col & (kTile - 1))] = o[e];
     }
   }
 }
 
-// Interpolation of one record from one half of a split row: the row's cells
-// are x[n] = Y0[n mod m] + w^(sign n) Y1[n mod m] (w = e^{2 pi i / nf1}, m = nf1 / 2),
-// and half h holds Yh; its share of the record's sum is added to the record's
+// Interpolation of one record from part h of a row split S ways: the row's
+// cells are x[n] = sum_h w^(sign h n) Y_h[n mod m] (w = e^{2 pi i / nf1}, m = nf1 / S),
+// and part h holds Y_h; its share of the record's sum is added to the record's
 // FP64 accumulator.
 template <typename T, int KWM, typename Tw>
-__device__ __forceinline__ void interp_row_half(const Rec<T> &rc, int idx, const cpx<T> *lds, const Tw &tv,
-                                                const Geom &g, int p, int row0, int h, int sign,
+__device__ __forceinline__ void interp_row_part(const Rec<T> &rc, int idx, const cpx<T> *lds, const Tw &tv,
+                                                const Geom &g, int p, int row0, int h, int mm, int sign,
                                                 double2 *__restrict__ vacc, T beta, T cc, T hw) {
-  const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2, mm = (nf1 >> 1) - 1;
+  const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2;
   int k = p - rc.iw;
   if (k < 0) k += g.P;
   if (k >= kw) return;
@@ -1280,7 +1285,7 @@ __device__ __forceinline__ void interp_row_half(const Rec<T> &rc, int idx, const
       const T kx = es_eval(rc.dx + (T)i, beta, cc, hw);
       cpx<T> f = lds[fftc::pad(cx & mm)];
       if (h) {
-        const cpx<T> w = tv[cx];
+        const cpx<T> w = tv[(h * cx) & (nf1 - 1)];
         f = fftc::mulw(f, w.x, (T)sign * w.y);
       }
       rr += f.x * kx;
@@ -1293,29 +1298,27 @@ __device__ __forceinline__ void interp_row_half(const Rec<T> &rc, int idx, const
 }
 
 // dirty2ms row pass for rows too long for two workgroups per CU: each row is
-// split into its even- and odd-index halves of length m = nf1 / 2 (radix-2
-// decimation in frequency of the inverse transform), one workgroup per half:
-// Yh = FFT_m over X[2k + h], so the pair of a row loads disjoint halves of H,
-// runs two m-point FFTs and adds its two shares to each record's accumulator.
-// Two such workgroups fit one CU, so one's memory phases overlap the other's FFT.
-template <typename T, int E, int KWM>
-__global__ void __launch_bounds__(MaxThreads<T>::value / 2) row_ifft_interp_half_kernel(
+// split S ways by the index of its inverse-transform input, k = S k' + h
+// (radix-S decimation in frequency), one workgroup per part: Y_h = FFT_m over
+// X[S k' + h], so the parts of a row load disjoint columns of H, run m-point
+// FFTs and add their shares to each record's accumulator.
+template <typename T, int E, int KWM, int S>
+__global__ void __launch_bounds__(MaxThreads<T>::value / S) row_ifft_interp_part_kernel(
     const cpx<T> *__restrict__ H, const Rec<T> *__restrict__ recs, const int *__restrict__ off,
     double2 *__restrict__ vacc, const cpx<T> *__restrict__ tw, Geom g, int p0, int sign) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ WalkTable wt;
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
-  const int nf1 = g.nf1, m = nf1 >> 1, N1 = g.N1, nt = blockDim.x;
-  // the two halves of a row are consecutive items: same XCD, adjacent in time
+  const int nf1 = g.nf1, m = nf1 / S, N1 = g.N1, nt = blockDim.x;
   const int item = xcd_group(blockIdx.x, gridDim.x);
-  const int row0 = item >> 1, h = item & 1, pl = blockIdx.y, p = p0 + pl;
-  const int lgm = 30 - __clz(nf1), lgt = 31 - __clz(g.nf2) + kTileLog2;
+  const int row0 = item / S, h = item % S, pl = blockIdx.y, p = p0 + pl;
+  const int lgm = 31 - __clz(m), lgt = 31 - __clz(g.nf2) + kTileLog2;
   const cpx<T> *hb = H + ((size_t)pl * g.nct * g.nf2 + row0) * kTile;
   cpx<T> v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int i = threadIdx.x + e * nt;  // < m = E * nt
-    const int col = image_index(2 * i + h, N1, nf1);
+    const int col = image_index(S * i + h, N1, nf1);
     v[e] = col >= 0 ? hb[((col >> kTileLog2) << lgt) + (col & (kTile - 1))] : fftc::mk((T)0, (T)0);
   }
   const auto tv = lds_twiddles<T>(lds + fftc::padded_size(m), tw, nf1);
@@ -1331,12 +1334,13 @@ __global__ void __launch_bounds__(MaxThreads<T>::value / 2) row_ifft_interp_half
     idx0 = walk_record(wt, threadIdx.x);
     r0 = recs[idx0];
   }
-  lds_fft_half<T, E, 16>(lds, lgm, sign, tv);
+  lds_fft_part<T, E, 16, S>(lds, lgm, sign, tv);
   const T beta = (T)g.beta, cc = (T)g.c, hw = (T)(0.5 * g.kw);
-  if (threadIdx.x < total) interp_row_half<T, KWM>(r0, idx0, lds, tv, g, p, row0, h, sign, vacc, beta, cc, hw);
+  if (threadIdx.x < total)
+    interp_row_part<T, KWM>(r0, idx0, lds, tv, g, p, row0, h, m - 1, sign, vacc, beta, cc, hw);
   for (int t = threadIdx.x + nt; t < total; t += nt) {
     const int idx = walk_record(wt, t);
-    interp_row_half<T, KWM>(recs[idx], idx, lds, tv, g, p, row0, h, sign, vacc, beta, cc, hw);
+    interp_row_part<T, KWM>(recs[idx], idx, lds, tv, g, p, row0, h, m - 1, sign, vacc, beta, cc, hw);
   }
 }
 
@@ -1677,7 +1681,7 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
   // pruned LDS FFT: power-of-two grids whose rows and columns fit one
   // workgroup's LDS, with the image at most half the grid on each axis
   g.fft_lds = 0;
-  g.rpw = g.cpw = g.nct = g.lds_row_bytes = g.lds_col_bytes = g.rhalf = g.lds_rowh_bytes = 0;
+  g.rpw = g.cpw = g.nct = g.lds_row_bytes = g.lds_col_bytes = g.rsplit = g.lds_rowp_bytes = 0;
   auto pow2 = [](int n) { return n > 0 && (n & (n - 1)) == 0; };
   const size_t cs = dbl ? 16 : 8;
   const size_t lds_max = tn.lds_bytes > 0 ? (size_t)tn.lds_bytes : size_t(160) << 10;
@@ -1697,10 +1701,12 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
       g.nct = (g.N1 + kTile - 1) / kTile;
       g.lds_row_bytes = (int)rb;
       g.lds_col_bytes = (int)cb;
-      // rows too long for two workgroups per CU run as two half-length halves
-      const size_t hb = (size_t)(fftc::padded_size(g.nf1 / 2) + fftc::twiddle_entries(g.nf1)) * cs;
-      g.rhalf = (rpw == 1 && rb + 2048 > lds_max / 2 && g.nf1 >= 4096) ? 1 : 0;
-      g.lds_rowh_bytes = (int)hb;
+      // rows too long for two workgroups per CU run split kRowSplit ways
+      if (rpw == 1 && rb + 2048 > lds_max / 2 && g.nf1 >= 4096) {
+        g.rsplit = kRowSplit;
+        g.lds_rowp_bytes =
+            (int)((size_t)(fftc::padded_size(g.nf1 / kRowSplit) + fftc::twiddle_entries(g.nf1)) * cs);
+      }
     }
   }
   // planes per batch from the memory budget: full planes (rocFFT path) or, on
@@ -1870,6 +1876,17 @@ int allow_lds(K kernel, int bytes) {
   return kOk;
 }
 
+template <typename T, int S>
+int allow_parts(const Geom &g) {
+  CURIG_TRY(allow_lds(spread_row_fft_part_kernel<T, 16, 8, S>, g.lds_rowp_bytes));
+  CURIG_TRY(allow_lds(spread_row_fft_part_kernel<T, 16, 16, S>, g.lds_rowp_bytes));
+  CURIG_TRY(allow_lds(spread_row_fft_part_kernel<T, 16, kMaxKernelWidth, S>, g.lds_rowp_bytes));
+  CURIG_TRY(allow_lds(row_ifft_interp_part_kernel<T, 16, 8, S>, g.lds_rowp_bytes));
+  CURIG_TRY(allow_lds(row_ifft_interp_part_kernel<T, 16, 16, S>, g.lds_rowp_bytes));
+  CURIG_TRY(allow_lds(row_ifft_interp_part_kernel<T, 16, kMaxKernelWidth, S>, g.lds_rowp_bytes));
+  return kOk;
+}
+
 template <typename T>
 int allow_kernels(const Geom &g) {
   if (!g.fft_lds) return kOk;
@@ -1879,12 +1896,8 @@ int allow_kernels(const Geom &g) {
   CURIG_TRY(allow_lds(row_ifft_interp_kernel<T, 16, 8>, g.lds_row_bytes));
   CURIG_TRY(allow_lds(row_ifft_interp_kernel<T, 16, 16>, g.lds_row_bytes));
   CURIG_TRY(allow_lds(row_ifft_interp_kernel<T, 16, kMaxKernelWidth>, g.lds_row_bytes));
-  CURIG_TRY(allow_lds(spread_row_fft_half_kernel<T, 16, 8>, g.lds_rowh_bytes));
-  CURIG_TRY(allow_lds(spread_row_fft_half_kernel<T, 16, 16>, g.lds_rowh_bytes));
-  CURIG_TRY(allow_lds(spread_row_fft_half_kernel<T, 16, kMaxKernelWidth>, g.lds_rowh_bytes));
-  CURIG_TRY(allow_lds(row_ifft_interp_half_kernel<T, 16, 8>, g.lds_rowh_bytes));
-  CURIG_TRY(allow_lds(row_ifft_interp_half_kernel<T, 16, 16>, g.lds_rowh_bytes));
-  CURIG_TRY(allow_lds(row_ifft_interp_half_kernel<T, 16, kMaxKernelWidth>, g.lds_rowh_bytes));
+  if (g.rsplit == 2) CURIG_TRY((allow_parts<T, 2>(g)));
+  if (g.rsplit == 4) CURIG_TRY((allow_parts<T, 4>(g)));
   CURIG_TRY(allow_lds(col_fwd_kernel<T, 16>, g.lds_col_bytes));
   CURIG_TRY(allow_lds(col_inv_kernel<T, 16>, g.lds_col_bytes));
   return kOk;
@@ -2055,20 +2068,42 @@ void launch_spread_row(const Rec<T> *recs, const int *off, cpx<T> *H, const cpx<
       recs, off, H, tw1, g, p0);
 }
 
+template <typename T, int S>
+void launch_spread_parts(const Rec<T> *recs, const int *off, cpx<T> *H, const cpx<T> *tw1, const Geom &g, int p0,
+                         int nb, hipStream_t st) {
+  const dim3 grd(S * g.nf2, nb);
+  const int nt = g.nf1 / (16 * S), lb = g.lds_rowp_bytes;
+  if (g.kw <= 8)
+    spread_row_fft_part_kernel<T, 16, 8, S><<<grd, nt, lb, st>>>(recs, off, H, tw1, g, p0);
+  else if (g.kw <= 16)
+    spread_row_fft_part_kernel<T, 16, 16, S><<<grd, nt, lb, st>>>(recs, off, H, tw1, g, p0);
+  else
+    spread_row_fft_part_kernel<T, 16, kMaxKernelWidth, S><<<grd, nt, lb, st>>>(recs, off, H, tw1, g, p0);
+}
+
+template <typename T, int S>
+void launch_interp_parts(const Rec<T> *recs, const int *off, const cpx<T> *H, double2 *vacc, const cpx<T> *tw1,
+                         const Geom &g, int p0, int nb, hipStream_t st) {
+  const dim3 grd(S * g.nf2, nb);
+  const int nt = g.nf1 / (16 * S), lb = g.lds_rowp_bytes;
+  if (g.kw <= 8)
+    row_ifft_interp_part_kernel<T, 16, 8, S><<<grd, nt, lb, st>>>(H, recs, off, vacc, tw1, g, p0, -1);
+  else if (g.kw <= 16)
+    row_ifft_interp_part_kernel<T, 16, 16, S><<<grd, nt, lb, st>>>(H, recs, off, vacc, tw1, g, p0, -1);
+  else
+    row_ifft_interp_part_kernel<T, 16, kMaxKernelWidth, S><<<grd, nt, lb, st>>>(H, recs, off, vacc, tw1, g, p0, -1);
+}
+
 // ms2dirty, one batch on the LDS path: spread + row FFT -> H (column-tiled)
 template <typename T>
 void lds_spread_rows(const Rec<T> *recs, const int *off, const Geom &g, int p0, int nb, cpx<T> *H,
                      const cpx<T> *tw1, hipStream_t st) {
-  if (g.rhalf) {
-    const dim3 grd(2 * g.nf2, nb);
-    const int nt = g.nf1 / 32;
-    if (g.kw <= 8)
-      spread_row_fft_half_kernel<T, 16, 8><<<grd, nt, g.lds_rowh_bytes, st>>>(recs, off, H, tw1, g, p0);
-    else if (g.kw <= 16)
-      spread_row_fft_half_kernel<T, 16, 16><<<grd, nt, g.lds_rowh_bytes, st>>>(recs, off, H, tw1, g, p0);
-    else
-      spread_row_fft_half_kernel<T, 16, kMaxKernelWidth><<<grd, nt, g.lds_rowh_bytes, st>>>(recs, off, H, tw1, g,
-                                                                                           p0);
+  if (g.rsplit == 2) {
+    launch_spread_parts<T, 2>(recs, off, H, tw1, g, p0, nb, st);
+    return;
+  }
+  if (g.rsplit == 4) {
+    launch_spread_parts<T, 4>(recs, off, H, tw1, g, p0, nb, st);
     return;
   }
   if (g.kw <= 8)
@@ -2099,16 +2134,12 @@ void lds_cols_backward(const Geom &g, int p0, int nb, const cpx<T> *gT, cpx<T> *
 template <typename T>
 void lds_row_interp(const Rec<T> *recs, const int *off, const cpx<T> *H, double2 *vacc, const cpx<T> *tw1,
                     const Geom &g, int p0, int nb, hipStream_t st) {
-  if (g.rhalf) {
-    const dim3 grd(2 * g.nf2, nb);
-    const int nt = g.nf1 / 32;
-    if (g.kw <= 8)
-      row_ifft_interp_half_kernel<T, 16, 8><<<grd, nt, g.lds_rowh_bytes, st>>>(H, recs, off, vacc, tw1, g, p0, -1);
-    else if (g.kw <= 16)
-      row_ifft_interp_half_kernel<T, 16, 16><<<grd, nt, g.lds_rowh_bytes, st>>>(H, recs, off, vacc, tw1, g, p0, -1);
-    else
-      row_ifft_interp_half_kernel<T, 16, kMaxKernelWidth><<<grd, nt, g.lds_rowh_bytes, st>>>(H, recs, off, vacc,
-                                                                                            tw1, g, p0, -1);
+  if (g.rsplit == 2) {
+    launch_interp_parts<T, 2>(recs, off, H, vacc, tw1, g, p0, nb, st);
+    return;
+  }
+  if (g.rsplit == 4) {
+    launch_interp_parts<T, 4>(recs, off, H, vacc, tw1, g, p0, nb, st);
     return;
   }
   const dim3 grd(g.nf2 / g.rpw, nb);

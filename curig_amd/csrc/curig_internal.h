@@ -87,7 +87,9 @@ struct Geom {
   int fft_lds;
   int rpw;            // grid rows per workgroup in the row pass (rpw * nf1 = 16 * threads)
   int cpw;            // grid columns per workgroup in the column pass (cpw * nf2 = 16 * threads)
-  int nct;            // column tiles of a plane's row-transformed layout (ceil(N1 / kTile))
+  int nct;            // column tiles of a plane's row-transformed layout (nslot / kTile, rounded up)
+  int nslot;          // column slots of that layout: N1, or nf1 / 2 with split rows
+  int slot_lg;        // split rows: log2 of the slots per row part (nf1 / (2 rsplit))
   int lds_row_bytes;  // dynamic LDS of the row-pass workgroup
   int lds_col_bytes;  // dynamic LDS of the column-pass workgroup
   int rsplit;         // > 1: rows run as rsplit workgroups of nf1 / rsplit (too long for two per CU)

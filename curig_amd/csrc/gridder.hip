@@ -1018,11 +1018,31 @@ __device__ __forceinline__ void interp_rows(const Rec<T> &rc, int idx, const cpx
 
 
 // The intermediate planes live in a column-tiled layout: element (plane k,
-// grid row k2, image column col) at ((k * nct + col / kTile) * nf2 + k2) * kTile
-// + col % kTile, so a row's kTile-column chunks are contiguous for the row
+// grid row k2, column slot s) at ((k * nct + s / kTile) * nf2 + k2) * kTile
+// + s % kTile, so a row's kTile-column chunks are contiguous for the row
 // kernels and a column's elements are kTile apart for the column kernels (the
 // kTile columns of a tile run on one XCD, whose L2 serves the shared sectors).
 constexpr int kTile = 1 << kTileLog2;
+
+// Column slots of the tiled layout.  With whole rows the slot of image column
+// col is col.  With rows split S ways, the columns of part h (grid index h mod
+// S, col = S i + c_h with c_h = (h + N1/2) mod S) take slots h M + i, M = nf1 /
+// (2 S) >= ceil(N1 / S), so a part writes whole kTile-column chunks; slots whose
+// i is past the part's last column are unused (-1 here).
+__device__ __forceinline__ int slot_column(int s, const Geom &g) {
+  if (g.rsplit <= 1) return s < g.N1 ? s : -1;
+  if (s >= g.nslot) return -1;
+  const int S = g.rsplit, h = s >> g.slot_lg, i = s & ((1 << g.slot_lg) - 1);
+  const int col = S * i + ((h + g.N1 / 2) & (S - 1));
+  return col < g.N1 ? col : -1;
+}
+
+// whether a column workgroup's slots [s0, s0 + cpw) hold any image column
+__device__ __forceinline__ bool slots_used(int s0, const Geom &g) {
+  for (int c = 0; c < g.cpw; ++c)
+    if (slot_column(s0 + c, g) >= 0) return true;
+  return false;
+}
 
 // Workgroup id -> work item such that each XCD (workgroups are dispatched to
 // the 8 XCDs round robin) takes one contiguous range of items.
@@ -1220,10 +1240,11 @@ __global__ void __launch_bounds__(MaxThreads<T>::value / S) spread_row_fft_part_
   const int N1 = g.N1, h1 = N1 / 2, c0 = (h + h1) % S, nout = (N1 - c0 + S - 1) / S;
   const int lgt = 31 - __clz(g.nf2) + kTileLog2;
   cpx<T> *hb = H + ((size_t)pl * g.nct * g.nf2 + row0) * kTile;
+  const int hs = h << g.slot_lg;  // this part's first column slot
   if (total == 0) {
     for (int i = threadIdx.x; i < nout; i += nt) {
-      const int col = S * i + c0;
-      hb[((col >> kTileLog2) << lgt) + (col & (kTile - 1))] = fftc::mk((T)0, (T)0);
+      const int sl = hs + i;
+      hb[((sl >> kTileLog2) << lgt) + (sl & (kTile - 1))] = fftc::mk((T)0, (T)0);
     }
     return;
   }
@@ -1255,8 +1276,8 @@ __global__ void __launch_bounds__(MaxThreads<T>::value / S) spread_row_fft_part_
   for (int e = 0; e < NO; ++e) {
     const int i = tid + e * nt;
     if (i < nout) {
-      const int col = S * i + c0;
-      hb[((col >> kTileLog2) << lgt) + (col & (kTile - 1))] = o[e];
+      const int sl = hs + i;
+      hb[((sl >> kTileLog2) << lgt) + (sl & (kTile - 1))] = o[e];
     }
   }
 }
@@ -1318,8 +1339,8 @@ __global__ void __launch_bounds__(MaxThreads<T>::value / S) row_ifft_interp_part
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int i = threadIdx.x + e * nt;  // < m = E * nt
-    const int col = image_index(S * i + h, N1, nf1);
-    v[e] = col >= 0 ? hb[((col >> kTileLog2) << lgt) + (col & (kTile - 1))] : fftc::mk((T)0, (T)0);
+    const int col = image_index(S * i + h, N1, nf1), sl = (h << g.slot_lg) + col / S;
+    v[e] = col >= 0 ? hb[((sl >> kTileLog2) << lgt) + (sl & (kTile - 1))] : fftc::mk((T)0, (T)0);
   }
   const auto tv = lds_twiddles<T>(lds + fftc::padded_size(m), tw, nf1);
   if (threadIdx.x < 64) walk_table_build(wt, off, g, p, row0, 1);
@@ -1365,8 +1386,9 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) col_fwd_kernel(const cpx
                                                                       int p0, int nb, int first) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
-  const int cpw = g.cpw, nf2 = g.nf2, N1 = g.N1, N2 = g.N2, nt = blockDim.x;
-  const int col0 = xcd_group(blockIdx.x, gridDim.x) * cpw;
+  const int cpw = g.cpw, nf2 = g.nf2, N2 = g.N2, nt = blockDim.x;
+  const int s0 = xcd_group(blockIdx.x, gridDim.x) * cpw;  // first column slot
+  if (!slots_used(s0, g)) return;
   const int lg = 31 - __clz(nf2), lgt = lg + kTileLog2;
   const int nout = cpw * N2;
   double ar[8], ai[8], zp[8];
@@ -1375,8 +1397,8 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) col_fwd_kernel(const cpx
     ar[e] = ai[e] = zp[e] = 0.0;
     const int o = threadIdx.x + e * nt;
     if (o < nout) {
-      const int ci = o / N2, row = o - ci * N2;
-      if (col0 + ci < N1) zp[e] = pixel_zp(row, col0 + ci, g);
+      const int ci = o / N2, row = o - ci * N2, col = slot_column(s0 + ci, g);
+      if (col >= 0) zp[e] = pixel_zp(row, col, g);
     }
   }
   const auto tv = lds_twiddles<T>(lds + fftc::padded_size(cpw * nf2), tw, nf2);
@@ -1390,9 +1412,9 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) col_fwd_kernel(const cpx
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = tid + e * nt;
-      const int ci = i >> lg, k2 = i & (nf2 - 1), col = col0 + ci;
-      v[e] = col < N1 ? hk[((col >> kTileLog2) << lgt) + (k2 << kTileLog2) + (col & (kTile - 1))]
-                      : fftc::mk((T)0, (T)0);
+      const int ci = i >> lg, k2 = i & (nf2 - 1), sl = s0 + ci;
+      v[e] = slot_column(sl, g) >= 0 ? hk[((sl >> kTileLog2) << lgt) + (k2 << kTileLog2) + (sl & (kTile - 1))]
+                                     : fftc::mk((T)0, (T)0);
     }
     __syncthreads();  // the previous plane's LDS reads are done
 #pragma unroll
@@ -1422,8 +1444,8 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) col_fwd_kernel(const cpx
   for (int e = 0; e < 8; ++e) {
     const int o = threadIdx.x + e * nt;
     if (o < nout) {
-      const int ci = o / N2, row = o - ci * N2, col = col0 + ci;
-      if (col < N1) {
+      const int ci = o / N2, row = o - ci * N2, col = slot_column(s0 + ci, g);
+      if (col >= 0) {
         const size_t a = (size_t)col * N2 + row;
         double2 r = make_double2(ar[e], ai[e]);
         if (!first) {
@@ -1449,8 +1471,9 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) col_inv_kernel(const cpx
                                                                       int p0, int nb) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
-  const int cpw = g.cpw, nf2 = g.nf2, N1 = g.N1, N2 = g.N2, nt = blockDim.x;
-  const int col0 = xcd_group(blockIdx.x, gridDim.x) * cpw;
+  const int cpw = g.cpw, nf2 = g.nf2, N2 = g.N2, nt = blockDim.x;
+  const int s0 = xcd_group(blockIdx.x, gridDim.x) * cpw;  // first column slot
+  if (!slots_used(s0, g)) return;
   const int lg = 31 - __clz(nf2);
   const int nout = cpw * N2, nzero = cpw * (nf2 - N2), h2 = N2 / 2;
   cpx<T> gv[8];
@@ -1461,8 +1484,8 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) col_inv_kernel(const cpx
     zp[e] = 0.0;
     const int o = threadIdx.x + e * nt;
     if (o < nout) {
-      const int ci = o / N2, row = o - ci * N2, col = col0 + ci;
-      if (col < N1) {
+      const int ci = o / N2, row = o - ci * N2, col = slot_column(s0 + ci, g);
+      if (col >= 0) {
         gv[e] = gT[(size_t)col * N2 + row];
         zp[e] = pixel_zp(row, col, g);
       }
@@ -1502,8 +1525,8 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) col_inv_kernel(const cpx
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = tid2 + e * nt;
-      const int ci = i >> lg, k2 = i & (nf2 - 1), col = col0 + ci;
-      if (col < N1) hk[((col >> kTileLog2) << (lg + kTileLog2)) + (k2 << kTileLog2) + (col & (kTile - 1))] = v[e];
+      const int ci = i >> lg, k2 = i & (nf2 - 1), sl = s0 + ci;
+      if (slot_column(sl, g) >= 0) hk[((sl >> kTileLog2) << (lg + kTileLog2)) + (k2 << kTileLog2) + (sl & (kTile - 1))] = v[e];
     }
   }
 }
@@ -1681,7 +1704,7 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
   // pruned LDS FFT: power-of-two grids whose rows and columns fit one
   // workgroup's LDS, with the image at most half the grid on each axis
   g.fft_lds = 0;
-  g.rpw = g.cpw = g.nct = g.lds_row_bytes = g.lds_col_bytes = g.rsplit = g.lds_rowp_bytes = 0;
+  g.rpw = g.cpw = g.nct = g.nslot = g.slot_lg = g.lds_row_bytes = g.lds_col_bytes = g.rsplit = g.lds_rowp_bytes = 0;
   auto pow2 = [](int n) { return n > 0 && (n & (n - 1)) == 0; };
   const size_t cs = dbl ? 16 : 8;
   const size_t lds_max = tn.lds_bytes > 0 ? (size_t)tn.lds_bytes : size_t(160) << 10;
@@ -1698,15 +1721,18 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
       g.fft_lds = 1;
       g.rpw = rpw;
       g.cpw = cpw;
-      g.nct = (g.N1 + kTile - 1) / kTile;
+      g.nslot = g.N1;
       g.lds_row_bytes = (int)rb;
       g.lds_col_bytes = (int)cb;
       // rows too long for two workgroups per CU run split kRowSplit ways
       if (rpw == 1 && rb + 2048 > lds_max / 2 && g.nf1 >= 4096) {
         g.rsplit = kRowSplit;
+        g.nslot = g.nf1 / 2;
+        g.slot_lg = 31 - __builtin_clz(g.nf1 / (2 * kRowSplit));
         g.lds_rowp_bytes =
             (int)((size_t)(fftc::padded_size(g.nf1 / kRowSplit) + fftc::twiddle_entries(g.nf1)) * cs);
       }
+      g.nct = (g.nslot + kTile - 1) / kTile;
     }
   }
   // planes per batch from the memory budget: full planes (rocFFT path) or, on
@@ -2114,7 +2140,7 @@ void lds_spread_rows(const Rec<T> *recs, const int *off, const Geom &g, int p0, 
     launch_spread_row<T, kMaxKernelWidth>(recs, off, H, tw1, g, p0, nb, st);
 }
 
-inline int col_groups(const Geom &g) { return (g.N1 + g.cpw - 1) / g.cpw; }
+inline int col_groups(const Geom &g) { return (g.nslot + g.cpw - 1) / g.cpw; }
 
 // ms2dirty, one batch on the LDS path: column FFT + w-accumulation into accT
 template <typename T>

@@ -21,7 +21,8 @@ EXPORTS = [
     "ms2dirty_1", "ms2dirty_2", "dirty2ms_1", "dirty2ms_2",
     "ms2dirty_f32", "dirty2ms_f32", "curig_ms2dirty_dev", "curig_dirty2ms_dev",
     "curig_ms2dirty_multi_dev", "curig_dirty2ms_multi_dev", "curig_explicit_ms2dirty_dev",
-    "curig_last_error", "curig_plan_info", "curig_kernel_fseries", "curig_gauss_legendre",
+    "curig_nifty_ms2dirty", "curig_nifty_dirty2ms",
+    "curig_last_error", "curig_plan_info", "curig_nifty_plan_info", "curig_kernel_fseries", "curig_gauss_legendre",
     "curig_set_tuning", "curig_set_fft_path", "curig_set_graph_mode", "curig_enable_stats", "curig_get_stats", "curig_release",
 ]
 # C++-linkage plan API (include/cugridder.h), exported with mangled names
@@ -79,6 +80,16 @@ def load():
     lib.curig_explicit_ms2dirty_dev.argtypes = [c_int, c_int, c_int, c_double, c_double, c_void_p, c_void_p,
                                                 c_void_p, c_int, c_long, c_void_p, c_void_p, c_int, c_void_p]
     lib.curig_explicit_ms2dirty_dev.restype = c_int
+    nifty = [c_int, c_int, _f64, _f64, c_void_p, c_void_p, c_void_p, c_int, c_int, c_double, c_double, c_int, c_int,
+             c_double, c_int, c_void_p, c_int]
+    for name in ("curig_nifty_ms2dirty", "curig_nifty_dirty2ms"):
+        f = getattr(lib, name)
+        f.argtypes = nifty
+        f.restype = c_int
+    lib.curig_nifty_plan_info.argtypes = [c_int, c_int, c_int, c_double, c_double, c_double, c_double, c_int, c_int,
+                                          c_int, c_double, c_double, c_int,
+                                          np.ctypeslib.ndpointer(np.int32, flags="C"), _f64]
+    lib.curig_nifty_plan_info.restype = c_int
     lib.curig_last_error.argtypes = []
     lib.curig_last_error.restype = ctypes.c_char_p
     lib.curig_plan_info.argtypes = [c_int, c_int, c_int, c_double, c_double, c_double, c_double, c_int, c_double,
@@ -115,6 +126,19 @@ def plan_info(nrow_valid, nx, ny, fov, freq, eps, sigma, sign, wmin, wmax, preci
     ints = np.zeros(9, dtype=np.int32)
     reals = np.zeros(8, dtype=np.float64)
     check(load().curig_plan_info(nrow_valid, nx, ny, fov, freq, eps, sigma, sign, wmin, wmax, precision, ints, reals))
+    keys_i = ["kw", "nf1", "nf2", "nf3", "planes_per_batch", "nbatch", "tile_x", "tile_y", "fft_lds"]
+    keys_r = ["beta", "c", "gamma", "h", "i_center", "o_center", "pixelsize_x", "pixelsize_y"]
+    d = {k: int(v) for k, v in zip(keys_i, ints)}
+    d.update({k: float(v) for k, v in zip(keys_r, reals)})
+    return d
+
+
+def nifty_plan_info(nrow_valid, nx, ny, pixsize_x, pixsize_y, freq, eps, nu, nv, do_wstacking, wmin, wmax,
+                    precision=0):
+    ints = np.zeros(9, dtype=np.int32)
+    reals = np.zeros(8, dtype=np.float64)
+    check(load().curig_nifty_plan_info(nrow_valid, nx, ny, pixsize_x, pixsize_y, freq, eps, nu, nv,
+                                       int(bool(do_wstacking)), wmin, wmax, precision, ints, reals))
     keys_i = ["kw", "nf1", "nf2", "nf3", "planes_per_batch", "nbatch", "tile_x", "tile_y", "fft_lds"]
     keys_r = ["beta", "c", "gamma", "h", "i_center", "o_center", "pixelsize_x", "pixelsize_y"]
     d = {k: int(v) for k, v in zip(keys_i, ints)}

@@ -24,8 +24,8 @@ Stats g_last_stats;
 
 struct HostPath {
   hipStream_t stream = nullptr;
-  void *buf[4] = {};
-  size_t cap[4] = {};
+  void *buf[5] = {};
+  size_t cap[5] = {};
 };
 std::vector<HostPath> g_host;
 
@@ -220,7 +220,8 @@ static int g_graph_mode = 1;
 
 static int multi(bool to_image, int nrow, int nchan, int nx, int ny, double fov, const double *freq,
                  const double *d_uvw, const void *d_vis, const void *d_wgt, const uint8_t *d_flag, const void *d_dirty,
-                 int per_channel, double eps, double sigma, int sign, int precision, void *stream) {
+                 int per_channel, double eps, double sigma, int sign, int precision, void *stream,
+                 const GridSpec *spec = nullptr) {
   CURIG_TRY(check_precision(precision));
   CURIG_TRY(precheck(nrow, nx, ny, eps, sigma));
   if (nchan <= 0 || !freq) {
@@ -238,6 +239,7 @@ static int multi(bool to_image, int nrow, int nchan, int nx, int ny, double fov,
     pb.flag_stride = nchan;
     if (per_channel) pb.dirty = (char *)const_cast<void *>(d_dirty) + cs * (size_t)nx * ny * c;
     pb.accumulate = to_image && !per_channel && c > 0;
+    if (spec) pb.spec = *spec;
     pbs[c] = pb;
   }
   return multi_device(to_image, pbs.data(), nchan, (hipStream_t)stream, &g_tuning, g_graph_mode, stats_ptr());
@@ -257,6 +259,91 @@ extern "C" int curig_dirty2ms_multi_dev(int nrow, int nchan, int nxdirty, int ny
                                         int precision, void *stream) {
   return multi(false, nrow, nchan, nxdirty, nydirty, fov, freq, d_uvw, d_vis, d_wgt, d_flag, d_dirty, per_channel,
                epsilon, sigma, sign, precision, stream);
+}
+
+// NIFTY-compatible host form: every channel of ms [nrow][nchan] gridded into
+// one image (or the image degridded into every channel), pixel sizes per axis,
+// optional grid size and mask, and the 2-D form without w-stacking.
+static int nifty_call(bool to_image, int nrow, int nchan, const double *uvw, const double *freq, void *ms,
+                      const void *wgt, const uint8_t *mask, int nx, int ny, double pixsize_x, double pixsize_y,
+                      int nu, int nv, double eps, int do_wstacking, void *dirty, int precision) {
+  CURIG_TRY(check_precision(precision));
+  CURIG_TRY(precheck(nrow, nx, ny, eps, 2.0));
+  if (nchan <= 0 || !freq || !dirty || (nrow > 0 && (!uvw || !ms))) {
+    set_error("nchan must be > 0, with frequency, image and (for nrow > 0) uvw and ms arrays");
+    return kErrArg;
+  }
+  if (!(pixsize_x > 0 && pixsize_y > 0 && std::isfinite(pixsize_x) && std::isfinite(pixsize_y))) {
+    set_error("pixel sizes (%g, %g) must be positive and finite", pixsize_x, pixsize_y);
+    return kErrNonFinite;
+  }
+  if ((nu == 0) != (nv == 0) || (nu != 0 && (nu <= nx || nv <= ny || (nu & 1) || (nv & 1)))) {
+    set_error("grid %d x %d must be even and larger than the image %d x %d (or 0 x 0)", nu, nv, nx, ny);
+    return kErrSize;
+  }
+  const double lmax = 0.5 * pixsize_x * nx, mmax = 0.5 * pixsize_y * ny;
+  if (!(lmax * lmax + mmax * mmax < 1.0)) {
+    set_error("field of view %g x %g rad reaches beyond the horizon (l^2 + m^2 >= 1)", 2 * lmax, 2 * mmax);
+    return kErrArg;
+  }
+  GridSpec spec;
+  spec.dx = pixsize_x;
+  spec.dy = pixsize_y;
+  spec.nu = nu;
+  spec.nv = nv;
+  spec.wstack = do_wstacking != 0;
+  std::lock_guard<std::mutex> lk(g_host_mu);
+  HostPath *h;
+  CURIG_TRY(host_path(&h));
+  const size_t cs = precision == 0 ? 16 : 8, rs = precision == 0 ? 8 : 4;
+  const size_t nr = (size_t)nrow, ns = nr * nchan, npix = (size_t)nx * ny;
+  void *d_uvw, *d_vis, *d_wgt = nullptr, *d_dirty, *d_flag = nullptr;
+  CURIG_TRY(host_buf(*h, 0, 24 * nr, &d_uvw));
+  CURIG_TRY(host_buf(*h, 1, cs * ns, &d_vis));
+  CURIG_TRY(host_buf(*h, 3, cs * npix, &d_dirty));
+  hipStream_t st = h->stream;
+  if (nrow > 0) CURIG_HIP(hipMemcpyAsync(d_uvw, uvw, 24 * nr, hipMemcpyHostToDevice, st));
+  if (wgt && nrow > 0) {
+    CURIG_TRY(host_buf(*h, 2, rs * ns, &d_wgt));
+    CURIG_HIP(hipMemcpyAsync(d_wgt, wgt, rs * ns, hipMemcpyHostToDevice, st));
+  }
+  std::vector<uint8_t> flag;
+  if (mask && nrow > 0) {
+    // mask != 0 keeps a sample; the gridder's flag != 0 drops it
+    flag.resize(ns);
+    for (size_t i = 0; i < ns; ++i) flag[i] = mask[i] == 0;
+    CURIG_TRY(host_buf(*h, 4, ns, &d_flag));
+    CURIG_HIP(hipMemcpyAsync(d_flag, flag.data(), ns, hipMemcpyHostToDevice, st));
+  }
+  if (to_image) {
+    if (nrow > 0) CURIG_HIP(hipMemcpyAsync(d_vis, ms, cs * ns, hipMemcpyHostToDevice, st));
+  } else {
+    CURIG_HIP(hipMemcpyAsync(d_dirty, dirty, cs * npix, hipMemcpyHostToDevice, st));
+  }
+  CURIG_TRY(multi(to_image, nrow, nchan, nx, ny, 0.0, freq, (const double *)d_uvw, d_vis, d_wgt,
+                  (const uint8_t *)d_flag, d_dirty, 0, eps, 2.0, -1, precision, st, &spec));
+  if (to_image)
+    CURIG_HIP(hipMemcpyAsync(dirty, d_dirty, cs * npix, hipMemcpyDeviceToHost, st));
+  else if (nrow > 0)
+    CURIG_HIP(hipMemcpyAsync(ms, d_vis, cs * ns, hipMemcpyDeviceToHost, st));
+  CURIG_HIP(hipStreamSynchronize(st));
+  return kOk;
+}
+
+extern "C" int curig_nifty_ms2dirty(int nrow, int nchan, const double *uvw, const double *freq, const void *ms,
+                                    const void *wgt, const uint8_t *mask, int nxdirty, int nydirty,
+                                    double pixsize_x, double pixsize_y, int nu, int nv, double epsilon,
+                                    int do_wstacking, void *dirty, int precision) {
+  return nifty_call(true, nrow, nchan, uvw, freq, const_cast<void *>(ms), wgt, mask, nxdirty, nydirty, pixsize_x,
+                    pixsize_y, nu, nv, epsilon, do_wstacking, dirty, precision);
+}
+
+extern "C" int curig_nifty_dirty2ms(int nrow, int nchan, const double *uvw, const double *freq, void *ms,
+                                    const void *wgt, const uint8_t *mask, int nxdirty, int nydirty,
+                                    double pixsize_x, double pixsize_y, int nu, int nv, double epsilon,
+                                    int do_wstacking, const void *dirty, int precision) {
+  return nifty_call(false, nrow, nchan, uvw, freq, ms, wgt, mask, nxdirty, nydirty, pixsize_x, pixsize_y, nu, nv,
+                    epsilon, do_wstacking, const_cast<void *>(dirty), precision);
 }
 
 extern "C" int curig_explicit_ms2dirty_dev(int nrow, int nxdirty, int nydirty, double fov, double freq,
@@ -285,6 +372,39 @@ extern "C" int curig_plan_info(int nrow_valid, int nxdirty, int nydirty, double 
   KernelOpts ko;
   CURIG_TRY(plan_geometry(nrow_valid, nxdirty, nydirty, fov, freq, epsilon, sigma, sign, wmin, wmax, precision == 0,
                           &g_tuning, &g, &ko));
+  if (ints) {
+    int v[9] = {g.kw, g.nf1, g.nf2, g.P, g.Pb, g.nbatch, g.T1, g.T2, g.fft_lds};
+    memcpy(ints, v, sizeof(v));
+  }
+  if (reals) {
+    double r[8] = {ko.beta, ko.c, g.gamma, g.h, g.C, g.D, g.dx, g.dy};
+    memcpy(reals, r, sizeof(r));
+  }
+  return kOk;
+}
+
+extern "C" int curig_nifty_plan_info(int nrow_valid, int nxdirty, int nydirty, double pixsize_x, double pixsize_y,
+                                     double freq, double epsilon, int nu, int nv, int do_wstacking, double wmin,
+                                     double wmax, int precision, int *ints, double *reals) {
+  CURIG_TRY(check_precision(precision));
+  if (nxdirty <= 0 || nydirty <= 0) {
+    set_error("image size %d x %d invalid", nxdirty, nydirty);
+    return kErrSize;
+  }
+  if (!(pixsize_x > 0 && pixsize_y > 0) || (nu == 0) != (nv == 0)) {
+    set_error("pixel sizes must be positive and the grid size given for both axes or neither");
+    return kErrArg;
+  }
+  GridSpec spec;
+  spec.dx = pixsize_x;
+  spec.dy = pixsize_y;
+  spec.nu = nu;
+  spec.nv = nv;
+  spec.wstack = do_wstacking != 0;
+  Geom g;
+  KernelOpts ko;
+  CURIG_TRY(plan_geometry(nrow_valid, nxdirty, nydirty, 0.0, freq, epsilon, 2.0, -1, wmin, wmax, precision == 0,
+                          &g_tuning, &g, &ko, &spec));
   if (ints) {
     int v[9] = {g.kw, g.nf1, g.nf2, g.P, g.Pb, g.nbatch, g.T1, g.T2, g.fft_lds};
     memcpy(ints, v, sizeof(v));

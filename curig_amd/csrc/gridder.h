@@ -8,6 +8,13 @@
 
 namespace curig {
 
+// Overrides of the fov / sigma defaults (the NIFTY gridder's arguments).
+struct GridSpec {
+  double dx = 0, dy = 0;  // pixel sizes in radians along x (u) and y (v); 0: fov / nx, fov / ny
+  int nu = 0, nv = 0;     // grid size along u and v (even, > image size); 0: from sigma
+  bool wstack = true;     // false: w taken as 0 and the image not divided by n
+};
+
 struct Problem {
   int nrow = 0, nx = 0, ny = 0;
   double fov = 0, freq = 0, eps = 1e-6, sigma = 2.0;
@@ -22,6 +29,7 @@ struct Problem {
   long flag_stride = 1;
   void *dirty = nullptr;        // complex [nx][ny]
   bool accumulate = false;      // ms2dirty adds into dirty instead of overwriting it
+  GridSpec spec;                // explicit pixel sizes / grid size / w handling (defaults: from fov, sigma)
 };
 
 // Filled on request: plan sizes and per-stage device times (ms, hipEvent).
@@ -53,7 +61,7 @@ int explicit_ms2dirty_device(const Problem &pb, long npix, const int64_t *pix, v
 // Host-only plan computation used by the tests (no device access).
 int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, double eps, double sigma,
                   int sign, double wmin_m, double wmax_m, bool dbl, const Tuning *tune, Geom *g,
-                  KernelOpts *ko);
+                  KernelOpts *ko, const GridSpec *spec = nullptr);
 
 void release_workspace();
 

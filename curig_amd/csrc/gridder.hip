@@ -500,7 +500,8 @@ __device__ __forceinline__ double phihat3(double zp, const double *__restrict__ 
 }
 
 // Corrections for pixel (row, col): 1 / (phihat1 phihat2 phihat3 n) and the
-// w-term phase C (z - D)  (deconv.cu:236-276, 371-399; ra_exec.cu:22-37).
+// w-term phase C (z - D)  (deconv.cu:236-276, 371-399; ra_exec.cu:22-37).  In
+// the 2-D form (divn == 0: every w is 0, so C == 0) the 1 / n is left out.
 __device__ __forceinline__ void pixel_correction(int row, int col, const Geom &g, const double *__restrict__ fk1,
                                                  const double *__restrict__ fk2, const double *__restrict__ q,
                                                  double &scale, double &phase) {
@@ -508,7 +509,7 @@ __device__ __forceinline__ void pixel_correction(int row, int col, const Geom &g
   pixel_n(row, col, g, nm1, n);
   double zp = (nm1 - g.D) * g.gamma * g.h;
   double ker = fk1[abs(col - g.N1 / 2)] * fk2[abs(row - g.N2 / 2)] * phihat3(zp, q, g.nquad);
-  scale = 1.0 / (ker * n);
+  scale = 1.0 / (g.divn ? ker * n : ker);
   phase = (nm1 - g.D) * g.C;
 }
 
@@ -708,6 +709,7 @@ __global__ void __launch_bounds__(256) explicit_kernel(const double *__restrict_
     int i = (int)(p / g.N1), k = (int)(p - (long)i * g.N1);
     double nm1, n;
     pixel_n(i, k, g, nm1, n);
+    if (!g.divn) nm1 = 0.0, n = 1.0;  // 2-D form: w ignored, no 1 / n
     double l = (double)(i - g.N2 / 2) * g.dx, m = (double)(k - g.N1 / 2) * g.dy;
     double re = 0.0, im = 0.0;
     for (long j = threadIdx.x; j < nrow; j += blockDim.x) {
@@ -1626,6 +1628,18 @@ struct Timer {
   }
 };
 
+// Pixel sizes (rad): dx = fov / nx pairs with u, dy = fov / ny with v
+// (src/astro/cugridder.cu:48-49), unless the caller gives both.
+void pixel_sizes(int nx, int ny, double fov, const GridSpec *spec, double &dx, double &dy) {
+  if (spec && (spec->dx != 0 || spec->dy != 0)) {
+    dx = spec->dx;
+    dy = spec->dy;
+    return;
+  }
+  dx = fov / 180.0 * M_PI / (double)nx;
+  dy = fov / 180.0 * M_PI / (double)ny;
+}
+
 int check_args(const Problem &pb) {
   if (pb.nrow < 0) {
     set_error("nrow %d < 0", pb.nrow);
@@ -1639,7 +1653,12 @@ int check_args(const Problem &pb) {
     set_error("sign must be +1 or -1 (got %d)", pb.sign);
     return kErrArg;
   }
-  if (!(std::isfinite(pb.fov) && pb.fov > 0) || !(std::isfinite(pb.freq) && pb.freq > 0)) {
+  const bool own_pix = pb.spec.dx != 0 || pb.spec.dy != 0;
+  if (own_pix && !(std::isfinite(pb.spec.dx) && pb.spec.dx > 0 && std::isfinite(pb.spec.dy) && pb.spec.dy > 0)) {
+    set_error("pixel sizes (%g, %g) must be positive and finite", pb.spec.dx, pb.spec.dy);
+    return kErrNonFinite;
+  }
+  if ((!own_pix && !(std::isfinite(pb.fov) && pb.fov > 0)) || !(std::isfinite(pb.freq) && pb.freq > 0)) {
     set_error("fov (%g) and freq (%g) must be positive and finite", pb.fov, pb.freq);
     return kErrNonFinite;
   }
@@ -1647,10 +1666,11 @@ int check_args(const Problem &pb) {
     set_error("NULL data pointer");
     return kErrArg;
   }
-  double dx = pb.fov * M_PI / 180.0 / pb.nx, dy = pb.fov * M_PI / 180.0 / pb.ny;
+  double dx, dy;
+  pixel_sizes(pb.nx, pb.ny, pb.fov, &pb.spec, dx, dy);
   double lmax = 0.5 * dx * pb.nx, mmax = 0.5 * dy * pb.ny;
   if (!(lmax * lmax + mmax * mmax < 1.0)) {
-    set_error("field of view %g deg reaches beyond the horizon (l^2 + m^2 >= 1)", pb.fov);
+    set_error("field of view %g x %g rad reaches beyond the horizon (l^2 + m^2 >= 1)", dx * pb.nx, dy * pb.ny);
     return kErrArg;
   }
   return kOk;
@@ -1659,7 +1679,18 @@ int check_args(const Problem &pb) {
 }  // namespace
 
 int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, double eps, double sigma, int sign,
-                  double wmin_m, double wmax_m, bool dbl, const Tuning *tune, Geom *gp, KernelOpts *kop) {
+                  double wmin_m, double wmax_m, bool dbl, const Tuning *tune, Geom *gp, KernelOpts *kop,
+                  const GridSpec *spec) {
+  const GridSpec sp = spec ? *spec : GridSpec();
+  const bool own_grid = sp.nu != 0 || sp.nv != 0;
+  if (own_grid) {
+    // a given grid size fixes the upsampling factor: the smaller of the two axes'
+    if (sp.nu <= nx || sp.nv <= ny || (sp.nu & 1) || (sp.nv & 1)) {
+      set_error("grid %d x %d must be even and larger than the image %d x %d", sp.nu, sp.nv, nx, ny);
+      return kErrSize;
+    }
+    sigma = std::min((double)sp.nu / nx, (double)sp.nv / ny);
+  }
   KernelOpts ko;
   int ier = setup_conv_opts(ko, eps, sigma);
   if (ier != kOk) return ier;
@@ -1670,12 +1701,13 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
   g.beta = ko.beta;
   g.c = ko.c;
   // src/astro/cugridder.cu:48-49 (N2 = nx pairs with u, N1 = ny with v)
-  g.dx = fov / 180.0 * M_PI / (double)nx;
-  g.dy = fov / 180.0 * M_PI / (double)ny;
+  pixel_sizes(nx, ny, fov, &sp, g.dx, g.dy);
   double fc = freq / kSpeedOfLight;
   g.uscale = fc * 2 * M_PI * g.dx;
   g.vscale = fc * 2 * M_PI * g.dy;
-  g.wscale = fc * sign * 2 * M_PI;
+  // 2-D form: every w is taken as 0 (w range 0, C = 0) and the 1 / n dropped
+  g.divn = sp.wstack ? 1 : 0;
+  g.wscale = sp.wstack ? fc * sign * 2 * M_PI : 0.0;
   // type-3 parameters (cugridder.cu:52-76)
   double l_min = -0.5 * g.dx * nx, m_min = -0.5 * g.dy * ny;
   double n_lm = std::sqrt(1.0 - l_min * l_min - m_min * m_min);
@@ -1692,8 +1724,12 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
     set_error("w range needs %d w planes (more than 1e7)", g.P);
     return kErrSize;
   }
-  g.nf1 = get_num_cells(ny, ko);
-  g.nf2 = get_num_cells(nx, ko);
+  g.nf1 = own_grid ? sp.nv : get_num_cells(ny, ko);
+  g.nf2 = own_grid ? sp.nu : get_num_cells(nx, ko);
+  if (g.nf1 < 2 * ko.kw || g.nf2 < 2 * ko.kw) {
+    set_error("grid %d x %d smaller than twice the kernel width %d", g.nf2, g.nf1, ko.kw);
+    return kErrSize;
+  }
   if ((double)g.nf1 * g.nf2 > 2.0e9) {
     set_error("grid %d x %d too large", g.nf1, g.nf2);
     return kErrSize;
@@ -1857,7 +1893,7 @@ int plan_all(const Problem *pbs, int n, hipStream_t st, const Tuning *tune, int 
     if (nvalid == 0) wmin = wmax = 0;
     const Problem &pb = pbs[i];
     CURIG_TRY(plan_geometry((int)nvalid, pb.nx, pb.ny, pb.fov, pb.freq, pb.eps, pb.sigma, pb.sign, wmin, wmax,
-                            pb.dbl, &tn, &cps[i].g, &cps[i].ko));
+                            pb.dbl, &tn, &cps[i].g, &cps[i].ko, &pb.spec));
     cps[i].nvalid = nvalid;
   }
   return kOk;
@@ -2496,8 +2532,8 @@ int explicit_ms2dirty_device(const Problem &pb, long npix, const int64_t *pix, v
   Geom g{};
   g.N1 = pb.ny;
   g.N2 = pb.nx;
-  g.dx = pb.fov / 180.0 * M_PI / (double)pb.nx;
-  g.dy = pb.fov / 180.0 * M_PI / (double)pb.ny;
+  pixel_sizes(pb.nx, pb.ny, pb.fov, &pb.spec, g.dx, g.dy);
+  g.divn = pb.spec.wstack ? 1 : 0;
   if (pix == nullptr) npix = (long)pb.nx * pb.ny;
   if (npix <= 0) return kOk;
   double fc2pi = 2 * M_PI * pb.freq / kSpeedOfLight;

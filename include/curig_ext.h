@@ -49,6 +49,27 @@ int curig_dirty2ms_multi_dev(int nrow, int nchan, int nxdirty, int nydirty, doub
                              const void *d_dirty, int per_channel, double epsilon, double sigma, int sign,
                              int precision, void *stream);
 
+/* NIFTY-compatible host form (the caller interface of the reference's
+ * python/curagridder/cursl.py:55-119, which passes only freq[0] and ignores
+ * rad_pix_y, nx, ny, do_wstacking and flags; these entries honour all of them).
+ * Host pointers; sign -1 and the NIFTY conventions:
+ *   dirty[i][k] = sum_{row,ch} mask wgt ms e^{+i 2pi f_ch/c (u l_i + v m_k - w (n-1))} / n
+ * with l_i = (i - nx/2) pixsize_x, m_k = (k - ny/2) pixsize_y.  dirty2ms is the
+ * adjoint and writes ms [nrow][nchan] (masked samples get 0).
+ *   ms, wgt, mask: [nrow][nchan] (wgt / mask may be NULL); mask != 0 keeps a sample.
+ *   nu, nv: grid size along u and v (even, larger than nx, ny), fixing the
+ *     upsampling factor to min(nu/nx, nv/ny); 0, 0: upsampling factor 2.
+ *   do_wstacking == 0: the 2-D transform, w taken as 0 and no division by n.
+ *   precision 0: ms / dirty complex128, wgt float64; 1: complex64 / float32.
+ * ms2dirty returns the complex image; the real part is NIFTY's dirty image. */
+int curig_nifty_ms2dirty(int nrow, int nchan, const double *uvw, const double *freq, const void *ms,
+                         const void *wgt, const uint8_t *mask, int nxdirty, int nydirty, double pixsize_x,
+                         double pixsize_y, int nu, int nv, double epsilon, int do_wstacking, void *dirty,
+                         int precision);
+int curig_nifty_dirty2ms(int nrow, int nchan, const double *uvw, const double *freq, void *ms, const void *wgt,
+                         const uint8_t *mask, int nxdirty, int nydirty, double pixsize_x, double pixsize_y, int nu,
+                         int nv, double epsilon, int do_wstacking, const void *dirty, int precision);
+
 /* FP64 direct sum on the GPU (the reference's explicit_gridder,
  * src/astro/precomp.cu:242-315, with the weight applied): out[q] (complex128) =
  * dirty pixel d_pix[q] = i*ny + k; with d_pix == NULL, every pixel (npix ignored).
@@ -67,6 +88,11 @@ const char *curig_last_error(void);
  * pixelsize_x, pixelsize_y. */
 int curig_plan_info(int nrow_valid, int nxdirty, int nydirty, double fov, double freq, double epsilon,
                     double sigma, int sign, double wmin, double wmax, int precision, int *ints, double *reals);
+/* Host-only: the same for the NIFTY form (sign -1; pixel sizes, grid size and
+ * do_wstacking as in curig_nifty_ms2dirty). */
+int curig_nifty_plan_info(int nrow_valid, int nxdirty, int nydirty, double pixsize_x, double pixsize_y,
+                          double freq, double epsilon, int nu, int nv, int do_wstacking, double wmin, double wmax,
+                          int precision, int *ints, double *reals);
 /* Host-only: correction factors phihat(k) (-1)^k, k = 0..nf/2, of the kernel
  * normalised to peak 1 (multiply by e^beta for the reference's unnormalised kernel). */
 int curig_kernel_fseries(double epsilon, double sigma, int nf, double *out);

@@ -159,3 +159,78 @@ void oracle_dft_dirty2ms(int nrow, int nx, int ny, double fov, double freq,
   }
   free(pl);
 }
+
+/* NIFTY form (the caller interface the reference's python/curagridder/cursl.py
+ * mirrors, with the arguments that file drops honoured): several channels,
+ * separate pixel sizes, a mask, and the 2-D form.
+ *   ms2dirty: dirty[i][k] = sum_{j,c} mask wgt ms e^{+i phi} / n
+ *   dirty2ms: ms[j][c]    = mask wgt sum_{i,k} dirty[i][k] / n e^{-i phi}
+ *   phi = 2 pi f_c/c (l_i u_j + m_k v_j - w_j (n_ik - 1))   (sign -1, cursl.py:75,108)
+ * l_i = (i - nx/2) dx, m_k = (k - ny/2) dy.  apply_w == 0 is NIFTY's
+ * do_wstacking=False: n - 1 taken as 0 and n as 1 (w ignored, no 1/n).
+ * ms, wgt, mask: [nrow][nchan]; wgt and mask may be NULL (mask != 0 keeps a
+ * sample).  to_image selects the direction; the output is overwritten. */
+void oracle_dft_nifty(int to_image, int nrow, int nchan, int nx, int ny, double dx, double dy, const double *freq,
+                      const double *uvw, double *ms, const double *wgt, const unsigned char *mask, int apply_w,
+                      double *dirty, int nthreads) {
+  set_threads(nthreads);
+  const int64_t npix = (int64_t)nx * ny;
+  double *pl = (double *)malloc(sizeof(double) * 4 * (size_t)npix);
+  for (int64_t p = 0; p < npix; ++p) {
+    int i = (int)(p / ny), k = (int)(p % ny);
+    double l, m, nm1, n;
+    pixel_lmn(i, k, nx, ny, dx, dy, &l, &m, &nm1, &n);
+    if (!apply_w) nm1 = 0.0, n = 1.0;
+    pl[4 * p] = l;
+    pl[4 * p + 1] = m;
+    pl[4 * p + 2] = nm1;
+    pl[4 * p + 3] = n;
+  }
+  if (to_image) {
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < npix; ++p) {
+      const double *q = pl + 4 * p;
+      double re = 0.0, im = 0.0;
+      for (int j = 0; j < nrow; ++j)
+        for (int c = 0; c < nchan; ++c) {
+          const int64_t s = (int64_t)j * nchan + c;
+          if (mask && !mask[s]) continue;
+          const double f = 2.0 * M_PI * freq[c] / ORACLE_C;
+          double ph = f * (q[0] * uvw[3 * j] + q[1] * uvw[3 * j + 1] - uvw[3 * j + 2] * q[2]);
+          double sn, cs;
+          sincos(ph, &sn, &cs);
+          double wt = wgt ? wgt[s] : 1.0;
+          double vr = ms[2 * s] * wt, vi = ms[2 * s + 1] * wt;
+          re += vr * cs - vi * sn;
+          im += vr * sn + vi * cs;
+        }
+      dirty[2 * p] = re / q[3];
+      dirty[2 * p + 1] = im / q[3];
+    }
+  } else {
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < nrow; ++j)
+      for (int c = 0; c < nchan; ++c) {
+        const int64_t s = (int64_t)j * nchan + c;
+        double re = 0.0, im = 0.0;
+        if (!mask || mask[s]) {
+          const double f = 2.0 * M_PI * freq[c] / ORACLE_C;
+          for (int64_t p = 0; p < npix; ++p) {
+            const double *q = pl + 4 * p;
+            double ph = -f * (q[0] * uvw[3 * j] + q[1] * uvw[3 * j + 1] - uvw[3 * j + 2] * q[2]);
+            double sn, cs;
+            sincos(ph, &sn, &cs);
+            double dr = dirty[2 * p] / q[3], di = dirty[2 * p + 1] / q[3];
+            re += dr * cs - di * sn;
+            im += dr * sn + di * cs;
+          }
+          double wt = wgt ? wgt[s] : 1.0;
+          re *= wt;
+          im *= wt;
+        }
+        ms[2 * s] = re;
+        ms[2 * s + 1] = im;
+      }
+  }
+  free(pl);
+}

@@ -273,6 +273,75 @@ def api_step_seconds(cfg):
 # ---------------------------------------------------------------------------
 # batched form: 64 flagged channels, one image per channel
 # ---------------------------------------------------------------------------
+def form_nifty(args, log):
+    """c2 through the NIFTY host form (curig_amd.cursl: 4 channels in one call,
+    host arrays in and out, FP64, pixel size fov/2048 on both axes, a 10 % random
+    mask) against the reference ABI's per-channel loop over the same unmasked
+    samples (ms2dirty_2 / dirty2ms_2 per channel, masked samples given weight 0).
+    Both include the host<->device copies."""
+    from curig_amd import _lib, cursl
+
+    lib = _lib.load()
+    cfg = make_config("c2", "fp64", args.eps)
+    uvw, vis, freqs = cfg["uvw"], cfg["vis"], cfg["freqs"]
+    nrow, nchan, npix, eps = uvw.shape[0], len(freqs), cfg["nx"], cfg["eps"]
+    rng = np.random.default_rng(7)
+    mask = (rng.random((nrow, nchan)) > 0.1).astype(np.uint8)
+    wgt = np.ascontiguousarray(cfg["wgt"])
+    nvalid = int(mask.sum(dtype=np.int64))
+    pix = cfg["fov"] * np.pi / 180 / npix
+    img = rng.standard_normal((npix, npix))
+    log(f"[bench] nifty: {nrow} rows x {nchan} channels, {nvalid} unmasked vis, {npix}^2, eps {eps}, fp64")
+
+    def step_nifty():
+        d = cursl.ms2dirty(uvw, freqs, vis, wgt, npix, npix, pix, pix, 0, 0, eps, True, mask=mask)
+        v = cursl.dirty2ms(uvw, freqs, img, wgt, pix, pix, 0, 0, eps, True, mask=mask)
+        return d, v
+
+    wz = np.ascontiguousarray(wgt * mask)
+    img_c = np.ascontiguousarray(img + 0j)
+
+    def step_loop():
+        d = np.zeros((npix, npix))
+        v = np.zeros((nrow, nchan), complex)
+        for c in range(nchan):
+            out = np.zeros((npix, npix), complex)
+            _lib.check(lib.ms2dirty_2(nrow, npix, npix, cfg["fov"], freqs[c], uvw, np.ascontiguousarray(vis[:, c]),
+                                      np.ascontiguousarray(wz[:, c]), out, eps, 2.0, -1))
+            d += out.real
+            vc = np.zeros(nrow, complex)
+            _lib.check(lib.dirty2ms_2(nrow, npix, npix, cfg["fov"], freqs[c], uvw, vc,
+                                      np.ascontiguousarray(wz[:, c]), img_c, eps, 2.0, -1))
+            v[:, c] = vc
+        return d, v
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            r = fn()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            r = fn()
+        return (time.perf_counter() - t0) / args.steps, r
+
+    t_n, (d_n, v_n) = timed(step_nifty)
+    t_l, (d_l, v_l) = timed(step_loop)
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    d_rel, v_rel = rel(d_n, d_l), rel(v_n, v_l)
+    value = nvalid / t_n / 1e6
+    log(f"[bench] nifty {t_n * 1e3:.1f} ms/step ({value:.2f} Mvis/s), reference-ABI channel loop "
+        f"{t_l * 1e3:.1f} ms/step ({nvalid / t_l / 1e6:.2f} Mvis/s); rel diff image {d_rel:.2e} vis {v_rel:.2e}")
+    return {
+        "metric": "Mvis/s ms2dirty+dirty2ms, NIFTY host form (PCIe-inclusive)",
+        "value": round(value, 4), "unit": "Mvis/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(t_n * 1e3, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (random-array uvw, Gaussian visibilities, 10% random mask, seeded)",
+        "config": {"workload": f"c2 via NIFTY host form: {nrow} rows x {nchan} channels (1.0-1.4 GHz), "
+                               f"{nvalid} unmasked vis, {npix}x{npix}, fov {cfg['fov']} deg, eps {eps}, fp64"},
+        "reference_abi_loop": {"ms_per_step": round(t_l * 1e3, 2), "value": round(nvalid / t_l / 1e6, 4)},
+        "nifty_vs_loop_rel": {"image": d_rel, "vis": v_rel},
+    }
+
+
 def form_batched(args, log):
     """64 channels linspace(1.0, 1.5) GHz of the 128-antenna 5 km array (256
     snapshots over +-3 h), per-channel flag fraction U[0, 0.4] as i.i.d. row
@@ -491,7 +560,7 @@ def main():
     ap.add_argument("--no-err", action="store_true")
     ap.add_argument("--no-api", action="store_true", help="skip the host-pointer (PCIe-inclusive) timing")
     ap.add_argument("--no-eps4", action="store_true", help="skip the epsilon 1e-4 run of the same workload")
-    ap.add_argument("--form", default="precision", choices=["precision", "batched", "diff"])
+    ap.add_argument("--form", default="precision", choices=["precision", "batched", "diff", "nifty"])
     ap.add_argument("--lr", type=float, default=0.02, help="Adam step of the diff form")
     ap.add_argument("--traffic", type=float, default=None,
                     help="HBM bytes per launch of the dominant kernel from a counter run (reported as given)")
@@ -504,7 +573,7 @@ def main():
         sys.exit(2)
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
     if args.form != "precision":
-        result = (form_batched if args.form == "batched" else form_diff)(args, log)
+        result = {"batched": form_batched, "diff": form_diff, "nifty": form_nifty}[args.form](args, log)
         print(json.dumps(result), flush=True)
         return
     cfg = make_config(args.config, args.precision, args.eps)
@@ -541,11 +610,12 @@ def main():
     launches = kb["launches"]
     mean = lambda key, d: sum(c[key] for c in calls[d]) / len(calls[d]) * 1e-3
     lds = bool(stats_by_ch[0]["fft_lds"])
-    kernels = {"spread_row_fft_kernel" if lds else "spread_tile_kernel": (kb["grid"], mean("t_grid", "m2d"))}
+    part = "_part" if stats_by_ch[0]["rsplit"] > 0 else ""  # rows split over several workgroups
+    kernels = {f"spread_row_fft{part}_kernel" if lds else "spread_tile_kernel": (kb["grid"], mean("t_grid", "m2d"))}
     if lds:
         kernels["col_fwd_kernel"] = (kb["col_fwd"], mean("t_fft", "m2d"))
         kernels["col_inv_kernel"] = (kb["col_inv"], mean("t_fft", "d2m"))
-        kernels["row_ifft_interp_kernel"] = (kb["row_interp"], mean("t_grid", "d2m"))
+        kernels[f"row_ifft_interp{part}_kernel"] = (kb["row_interp"], mean("t_grid", "d2m"))
     per_kernel = {k: {"bytes_per_launch": round(b / launches), "avg_launch_ms": round(t / launches * 1e3, 3),
                       "achieved_GBps": round(b / t / 1e9, 1), "frac": round(b / t / 1e9 / HBM_PEAK_GBS, 4)}
                   for k, (b, t) in kernels.items() if t > 0}

@@ -123,10 +123,10 @@ def check(code):
 
 
 def plan_info(nrow_valid, nx, ny, fov, freq, eps, sigma, sign, wmin, wmax, precision=0):
-    ints = np.zeros(9, dtype=np.int32)
+    ints = np.zeros(10, dtype=np.int32)
     reals = np.zeros(8, dtype=np.float64)
     check(load().curig_plan_info(nrow_valid, nx, ny, fov, freq, eps, sigma, sign, wmin, wmax, precision, ints, reals))
-    keys_i = ["kw", "nf1", "nf2", "nf3", "planes_per_batch", "nbatch", "tile_x", "tile_y", "fft_lds"]
+    keys_i = ["kw", "nf1", "nf2", "nf3", "planes_per_batch", "nbatch", "tile_x", "tile_y", "fft_lds", "rsplit"]
     keys_r = ["beta", "c", "gamma", "h", "i_center", "o_center", "pixelsize_x", "pixelsize_y"]
     d = {k: int(v) for k, v in zip(keys_i, ints)}
     d.update({k: float(v) for k, v in zip(keys_r, reals)})
@@ -135,11 +135,11 @@ def plan_info(nrow_valid, nx, ny, fov, freq, eps, sigma, sign, wmin, wmax, preci
 
 def nifty_plan_info(nrow_valid, nx, ny, pixsize_x, pixsize_y, freq, eps, nu, nv, do_wstacking, wmin, wmax,
                     precision=0):
-    ints = np.zeros(9, dtype=np.int32)
+    ints = np.zeros(10, dtype=np.int32)
     reals = np.zeros(8, dtype=np.float64)
     check(load().curig_nifty_plan_info(nrow_valid, nx, ny, pixsize_x, pixsize_y, freq, eps, nu, nv,
                                        int(bool(do_wstacking)), wmin, wmax, precision, ints, reals))
-    keys_i = ["kw", "nf1", "nf2", "nf3", "planes_per_batch", "nbatch", "tile_x", "tile_y", "fft_lds"]
+    keys_i = ["kw", "nf1", "nf2", "nf3", "planes_per_batch", "nbatch", "tile_x", "tile_y", "fft_lds", "rsplit"]
     keys_r = ["beta", "c", "gamma", "h", "i_center", "o_center", "pixelsize_x", "pixelsize_y"]
     d = {k: int(v) for k, v in zip(keys_i, ints)}
     d.update({k: float(v) for k, v in zip(keys_r, reals)})
@@ -147,10 +147,10 @@ def nifty_plan_info(nrow_valid, nx, ny, pixsize_x, pixsize_y, freq, eps, nu, nv,
 
 
 def stats():
-    ints = np.zeros(9, dtype=np.int32)
+    ints = np.zeros(10, dtype=np.int32)
     reals = np.zeros(8, dtype=np.float64)
     load().curig_get_stats(ints, reals)
-    keys_i = ["kw", "nf1", "nf2", "nf3", "planes_per_batch", "nbatch", "tile_x", "tile_y", "fft_lds"]
+    keys_i = ["kw", "nf1", "nf2", "nf3", "planes_per_batch", "nbatch", "tile_x", "tile_y", "fft_lds", "rsplit"]
     keys_r = ["nvis", "t_prep", "t_grid", "t_fft", "t_plane", "t_final", "t_total", "timed"]
     d = {k: int(v) for k, v in zip(keys_i, ints)}
     d.update({k: float(v) for k, v in zip(keys_r, reals)})

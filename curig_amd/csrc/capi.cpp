@@ -373,7 +373,7 @@ extern "C" int curig_plan_info(int nrow_valid, int nxdirty, int nydirty, double 
   CURIG_TRY(plan_geometry(nrow_valid, nxdirty, nydirty, fov, freq, epsilon, sigma, sign, wmin, wmax, precision == 0,
                           &g_tuning, &g, &ko));
   if (ints) {
-    int v[9] = {g.kw, g.nf1, g.nf2, g.P, g.Pb, g.nbatch, g.T1, g.T2, g.fft_lds};
+    int v[10] = {g.kw, g.nf1, g.nf2, g.P, g.Pb, g.nbatch, g.T1, g.T2, g.fft_lds, g.rsplit};
     memcpy(ints, v, sizeof(v));
   }
   if (reals) {
@@ -406,7 +406,7 @@ extern "C" int curig_nifty_plan_info(int nrow_valid, int nxdirty, int nydirty, d
   CURIG_TRY(plan_geometry(nrow_valid, nxdirty, nydirty, 0.0, freq, epsilon, 2.0, -1, wmin, wmax, precision == 0,
                           &g_tuning, &g, &ko, &spec));
   if (ints) {
-    int v[9] = {g.kw, g.nf1, g.nf2, g.P, g.Pb, g.nbatch, g.T1, g.T2, g.fft_lds};
+    int v[10] = {g.kw, g.nf1, g.nf2, g.P, g.Pb, g.nbatch, g.T1, g.T2, g.fft_lds, g.rsplit};
     memcpy(ints, v, sizeof(v));
   }
   if (reals) {
@@ -473,7 +473,7 @@ extern "C" int curig_enable_stats(int on) {
 extern "C" int curig_get_stats(int *ints, double *reals) {
   const Stats &s = g_last_stats;
   if (ints) {
-    int v[9] = {s.kw, s.nf1, s.nf2, s.P, s.Pb, s.nbatch, s.T1, s.T2, s.fft_lds};
+    int v[10] = {s.kw, s.nf1, s.nf2, s.P, s.Pb, s.nbatch, s.T1, s.T2, s.fft_lds, s.rsplit};
     memcpy(ints, v, sizeof(v));
   }
   if (reals) {
@@ -486,8 +486,8 @@ extern "C" int curig_get_stats(int *ints, double *reals) {
 extern "C" void curig_release(void) {
   std::lock_guard<std::mutex> lk(g_host_mu);
   for (auto &h : g_host) {
-    for (int i = 0; i < 4; ++i)
-      if (h.buf[i]) (void)hipFree(h.buf[i]);
+    for (void *b : h.buf)
+      if (b) (void)hipFree(b);
     if (h.stream) (void)hipStreamDestroy(h.stream);
   }
   g_host.clear();

@@ -36,6 +36,7 @@ struct Problem {
 struct Stats {
   int kw = 0, nf1 = 0, nf2 = 0, P = 0, Pb = 0, nbatch = 0, T1 = 0, T2 = 0;
   int fft_lds = 0;             // 1: pruned LDS FFT path, 0: rocFFT
+  int rsplit = 0;              // LDS path: parts per grid row in the row kernels (0: whole rows)
   long nvis = 0;               // unflagged rows gridded
   double beta = 0, gamma = 0, C = 0, D = 0;
   double t_prep = 0, t_grid = 0, t_fft = 0, t_plane = 0, t_final = 0, t_total = 0;

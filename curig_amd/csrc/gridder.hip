@@ -2226,6 +2226,7 @@ void fill_stats(Stats *sts, const ChanPlan &cp) {
   sts->T1 = g.T1;
   sts->T2 = g.T2;
   sts->fft_lds = g.fft_lds;
+  sts->rsplit = g.rsplit;
   sts->nvis = cp.nvalid;
   sts->beta = g.beta;
   sts->gamma = g.gamma;

@@ -82,9 +82,9 @@ int curig_explicit_ms2dirty_dev(int nrow, int nxdirty, int nydirty, double fov, 
 const char *curig_last_error(void);
 
 /* Host-only plan computation (no device access): for nrow_valid unflagged rows
- * whose w (metres) spans [wmin, wmax].  ints[9] = kw, nf1, nf2, nf3 (w planes),
+ * whose w (metres) spans [wmin, wmax].  ints[10] = kw, nf1, nf2, nf3 (w planes),
  * planes per batch, batches, tile width, tile height, FFT path (1: pruned LDS
- * FFT, 0: rocFFT); reals[8] = beta, c, gamma, h, i_center, o_center,
+ * FFT, 0: rocFFT), row split (parts per grid row, 0: whole rows); reals[8] = beta, c, gamma, h, i_center, o_center,
  * pixelsize_x, pixelsize_y. */
 int curig_plan_info(int nrow_valid, int nxdirty, int nydirty, double fov, double freq, double epsilon,
                     double sigma, int sign, double wmin, double wmax, int precision, int *ints, double *reals);
@@ -114,8 +114,9 @@ int curig_set_graph_mode(int on);
 /* 1: time the stages of later calls with hipEvents (adds synchronisation;
  * in graph mode only the total is timed). */
 int curig_enable_stats(int on);
-/* Stats of the last call on this process: ints[9] = kw, nf1, nf2, nf3, planes
- * per batch, batches, tile width, tile height, FFT path; reals[8] = rows gridded, t_prep,
+/* Stats of the last call on this process: ints[10] = kw, nf1, nf2, nf3, planes
+ * per batch, batches, tile width, tile height, FFT path (1 = LDS), row split
+ * (parts per grid row in the LDS row kernels, 0 = whole rows); reals[8] = rows gridded, t_prep,
  * t_grid (spread / interp), t_fft, t_plane (w accumulate / w fill), t_final,
  * t_total (ms), timed flag. */
 int curig_get_stats(int *ints, double *reals);

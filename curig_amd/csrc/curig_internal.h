@@ -75,6 +75,7 @@ struct Geom {
   int N1, N2;         // image: N1 = ny (fast), N2 = nx
   int nf1, nf2, P;    // grid sizes: nf1 (v), nf2 (u), w planes
   int kw;
+  int kww;            // kernel width along w: kw, or 1 in the 2-D form (one plane, weight 1)
   double beta, c;     // normalised ES kernel exp(beta (sqrt(1 - c d^2) - 1))
   double dx, dy;      // pixel sizes (rad): dx pairs with u / image rows, dy with v / columns
   int divn;           // 1: image = sum / n (w-stacking); 0: w taken as 0 and no 1/n (2-D transform)

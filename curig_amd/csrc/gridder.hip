@@ -120,6 +120,10 @@ __device__ __forceinline__ Coords row_coords(const double *__restrict__ uvw, lon
   c.ix = imod((int)sx, g.nf1);
   c.iy = imod((int)sy, g.nf2);
   c.iw = imod((int)sw, g.P);
+  if (g.kww == 1) {  // 2-D form: the single plane, kernel argument 0 (weight 1)
+    c.iw = 0;
+    c.dw = 0.0;
+  }
   return c;
 }
 
@@ -390,10 +394,10 @@ __global__ void __launch_bounds__(256) spread_tile_kernel(const Rec<T> *__restri
   for (int i = threadIdx.x; i < 2 * ncell; i += blockDim.x) acc[i] = (T)0;
   __syncthreads();
 
-  const int kw = g.kw;
+  const int kw = g.kw, kww = g.kww;
   const T beta = (T)g.beta, cc = (T)g.c, hw = (T)(0.5 * kw);
-  // source rows: first plane in [p-kw+1, p], first cell in [x0-kw+1, x0+wx-1] x [y0-kw+1, y0+wy-1]
-  walk_tiles(imod(p - kw + 1, g.P), kw, g.P, g.Pbin, g.nlayer, [&](int sb) {
+  // source rows: first plane in [p-kww+1, p], first cell in [x0-kw+1, x0+wx-1] x [y0-kw+1, y0+wy-1]
+  walk_tiles(imod(p - kww + 1, g.P), kww, g.P, g.Pbin, g.nlayer, [&](int sb) {
     walk_tiles(imod(y0 - kw + 1, g.nf2), wy + kw - 1, g.nf2, g.T2, g.nty, [&](int sty) {
       walk_tiles(imod(x0 - kw + 1, g.nf1), wx + kw - 1, g.nf1, g.T1, g.ntx, [&](int stx) {
         const int bin = sb * g.ntiles + sty * g.ntx + stx;
@@ -402,7 +406,7 @@ __global__ void __launch_bounds__(256) spread_tile_kernel(const Rec<T> *__restri
           const Rec<T> rc = recs[r];
           int k = p - rc.iw;
           if (k < 0) k += g.P;
-          if (k >= kw) continue;
+          if (k >= kww) continue;
           int rx = rc.ix - x0;
           if (rx < 0) rx += g.nf1;
           int ry = rc.iy - y0;
@@ -501,14 +505,15 @@ __device__ __forceinline__ double phihat3(double zp, const double *__restrict__ 
 
 // Corrections for pixel (row, col): 1 / (phihat1 phihat2 phihat3 n) and the
 // w-term phase C (z - D)  (deconv.cu:236-276, 371-399; ra_exec.cu:22-37).  In
-// the 2-D form (divn == 0: every w is 0, so C == 0) the 1 / n is left out.
+// the 2-D form (divn == 0: every w is 0, so C == 0; one plane, kww == 1) the
+// 1 / n and phihat3 are left out.
 __device__ __forceinline__ void pixel_correction(int row, int col, const Geom &g, const double *__restrict__ fk1,
                                                  const double *__restrict__ fk2, const double *__restrict__ q,
                                                  double &scale, double &phase) {
   double nm1, n;
   pixel_n(row, col, g, nm1, n);
   double zp = (nm1 - g.D) * g.gamma * g.h;
-  double ker = fk1[abs(col - g.N1 / 2)] * fk2[abs(row - g.N2 / 2)] * phihat3(zp, q, g.nquad);
+  double ker = fk1[abs(col - g.N1 / 2)] * fk2[abs(row - g.N2 / 2)] * (g.kww == 1 ? 1.0 : phihat3(zp, q, g.nquad));
   scale = 1.0 / (g.divn ? ker * n : ker);
   phase = (nm1 - g.D) * g.C;
 }
@@ -619,7 +624,7 @@ __global__ void __launch_bounds__(256) interp_kernel(const Rec<T> *__restrict__ 
                                                       const v2_t<T> *__restrict__ grid,
                                                       double2 *__restrict__ vacc, Geom g, int p0, int nb) {
   int ra0, ra1, rb0, rb1;
-  layer_ranges(off, g, imod(p0 - g.kw + 1, g.P), nb + g.kw - 1, ra0, ra1, rb0, rb1);
+  layer_ranges(off, g, imod(p0 - g.kww + 1, g.P), nb + g.kww - 1, ra0, ra1, rb0, rb1);
   const long n0 = ra1 - ra0, ntot = n0 + (rb1 - rb0);
   const size_t plane = (size_t)g.nf1 * g.nf2;
   const int kw = g.kw;
@@ -629,7 +634,7 @@ __global__ void __launch_bounds__(256) interp_kernel(const Rec<T> *__restrict__ 
     const Rec<T> rc = recs[r];
     T kx[KWM], ky[KWM];
     bool any = false;
-    for (int k = 0; k < kw; ++k) {
+    for (int k = 0; k < g.kww; ++k) {
       int q = rc.iw + k;
       if (q >= g.P) q -= g.P;
       if (q >= p0 && q < p0 + nb) any = true;
@@ -641,7 +646,7 @@ __global__ void __launch_bounds__(256) interp_kernel(const Rec<T> *__restrict__ 
       ky[i] = (i < kw) ? es_eval(rc.dy + (T)i, beta, cc, hw) : (T)0;
     }
     double sr = 0.0, si = 0.0;
-    for (int k = 0; k < kw; ++k) {
+    for (int k = 0; k < g.kww; ++k) {
       int q = rc.iw + k;
       if (q >= g.P) q -= g.P;
       if (q < p0 || q >= p0 + nb) continue;
@@ -779,8 +784,8 @@ __host__ __device__ inline int tiles_count(int a, int len, int n, int t, int nti
 }
 
 // Upper bound of the bins a row workgroup walks (any p, row0).
-__host__ __device__ inline int walk_bins_bound(int kw, int Pbin, int nlayer, int nrows, int T2, int nty) {
-  int nl = (kw - 1) / Pbin + 2, ns = (nrows + kw - 2) / T2 + 2;
+__host__ __device__ inline int walk_bins_bound(int kww, int kw, int Pbin, int nlayer, int nrows, int T2, int nty) {
+  int nl = (kww - 1) / Pbin + 2, ns = (nrows + kw - 2) / T2 + 2;
   if (nl > nlayer) nl = nlayer;
   if (ns > nty) ns = nty;
   return nl * ns;
@@ -790,8 +795,8 @@ __host__ __device__ inline int walk_bins_bound(int kw, int Pbin, int nlayer, int
 __device__ __forceinline__ void walk_table_build(WalkTable &wt, const int *__restrict__ off, const Geom &g, int p,
                                                  int row0, int nrows) {
   const int lane = threadIdx.x;
-  const int al = imod(p - g.kw + 1, g.P), as = imod(row0 - g.kw + 1, g.nf2);
-  const int nl = tiles_count(al, g.kw, g.P, g.Pbin, g.nlayer);
+  const int al = imod(p - g.kww + 1, g.P), as = imod(row0 - g.kw + 1, g.nf2);
+  const int nl = tiles_count(al, g.kww, g.P, g.Pbin, g.nlayer);
   const int ns = tiles_count(as, nrows + g.kw - 1, g.nf2, g.T2, g.nty);
   int nb = nl * ns;
   if (nb > kMaxWalk) nb = kMaxWalk;  // the plan keeps nb within kMaxWalk
@@ -948,7 +953,7 @@ __device__ __forceinline__ void spread_rows(const Rec<T> &rc, T *ldsf, const Geo
   const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2, rpw = g.rpw;
   int k = p - rc.iw;
   if (k < 0) k += g.P;
-  if (k >= kw) return;
+  if (k >= g.kww) return;
   int ry = rc.iy - row0;
   if (ry < 0) ry += nf2;
   if (!(ry < rpw || ry + kw - 1 >= nf2)) return;
@@ -985,7 +990,7 @@ __device__ __forceinline__ void interp_rows(const Rec<T> &rc, int idx, const cpx
   const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2, rpw = g.rpw;
   int k = p - rc.iw;
   if (k < 0) k += g.P;
-  if (k >= kw) return;
+  if (k >= g.kww) return;
   int ry = rc.iy - row0;
   if (ry < 0) ry += nf2;
   if (!(ry < rpw || ry + kw - 1 >= nf2)) return;
@@ -1188,7 +1193,7 @@ __device__ __forceinline__ void spread_row_part(const Rec<T> &rc, T *ldsf, const
   const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2;
   int k = p - rc.iw;
   if (k < 0) k += g.P;
-  if (k >= kw) return;
+  if (k >= g.kww) return;
   int jj = row0 - rc.iy;
   if (jj < 0) jj += nf2;
   if (jj >= kw) return;
@@ -1295,7 +1300,7 @@ __device__ __forceinline__ void interp_row_part(const Rec<T> &rc, int idx, const
   const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2;
   int k = p - rc.iw;
   if (k < 0) k += g.P;
-  if (k >= kw) return;
+  if (k >= g.kww) return;
   int jj = row0 - rc.iy;
   if (jj < 0) jj += nf2;
   if (jj >= kw) return;
@@ -1720,6 +1725,15 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
   g.D = o_min / 2.0;
   double S = std::fabs(o_min / 2.0);
   set_nhg_type3(S, X, ko, g.P, g.h, g.gamma);
+  g.kww = g.kw;
+  if (!sp.wstack) {
+    // 2-D form: one plane holding every row with weight 1 (w-DFT phase and
+    // phihat3 are 1), so the transform is the plain 2-D NUFFT
+    g.P = 1;
+    g.kww = 1;
+    g.h = 2 * M_PI;
+    g.gamma = 1.0;
+  }
   if (g.P > 10000000) {
     set_error("w range needs %d w planes (more than 1e7)", g.P);
     return kErrSize;
@@ -1795,7 +1809,7 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
   g.nty = (g.nf2 + g.T2 - 1) / g.T2;
   if (g.fft_lds) {
     // a row workgroup walks at most kMaxWalk bins: coarsen strips, then layers
-    while (walk_bins_bound(g.kw, g.Pbin, g.nlayer, g.rpw, g.T2, g.nty) > kMaxWalk) {
+    while (walk_bins_bound(g.kww, g.kw, g.Pbin, g.nlayer, g.rpw, g.T2, g.nty) > kMaxWalk) {
       if (g.T2 < g.nf2) {
         g.T2 = std::min(2 * g.T2, g.nf2);
         g.nty = (g.nf2 + g.T2 - 1) / g.T2;
@@ -2113,7 +2127,7 @@ void interp_dispatch(const Rec<T> *recs, const int *off, const v2_t<T> *grid, do
   if (nvalid <= 0) return;
   // the kernel reads its record ranges from the bin offsets; a batch touches at
   // most the records of (nb + kw - 1) planes' layers, rounded up to whole layers
-  const double frac = std::min(1.0, (double)(nb + g.kw - 1 + 2 * g.Pbin) / g.P);
+  const double frac = std::min(1.0, (double)(nb + g.kww - 1 + 2 * g.Pbin) / g.P);
   const int nbk = std::max(1, std::min(nblocks((long)(nvalid * frac) + 1), 4096));
   if (g.kw <= 8)
     interp_kernel<T, 8><<<nbk, 256, 0, st>>>(recs, off, grid, vacc, g, p0, nb);

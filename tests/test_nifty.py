@@ -114,14 +114,14 @@ def test_nifty_plan_grid_size(nx, ny, nu, nv):
 
 
 def test_nifty_plan_without_wstacking():
+    # the 2-D form: every w taken as 0, one plane (the plain 2-D NUFFT)
     nx, ny, dx, dy, eps = 128, 96, 2e-4, 2.5e-4, 1e-6
     got = _lib.nifty_plan_info(10, nx, ny, dx, dy, 1.2e9, eps, 0, 0, False, -500.0, 700.0)
-    _, kw, _, _ = R.setup_conv_opts(eps, 2.0)
-    lmin, mmin = -0.5 * dx * nx, -0.5 * dy * ny
-    S = abs((np.sqrt(1 - lmin ** 2 - mmin ** 2) - 1) / 2)
-    P, h, gam = R.set_nhg_type3(S, 0.0, kw, 2.0)   # every w taken as 0
+    want = _lib.nifty_plan_info(10, nx, ny, dx, dy, 1.2e9, eps, 0, 0, True, -500.0, 700.0)
     assert got["i_center"] == 0.0
-    assert (got["nf3"], got["h"]) == (P, pytest.approx(h, rel=1e-13))
+    assert (got["nf3"], got["nbatch"]) == (1, 1)
+    for k in ("kw", "nf1", "nf2", "beta", "c", "pixelsize_x", "pixelsize_y"):
+        assert got[k] == want[k], k
 
 
 def test_nifty_argument_errors():

@@ -21,12 +21,13 @@ EXPORTS = [
     "ms2dirty_1", "ms2dirty_2", "dirty2ms_1", "dirty2ms_2",
     "ms2dirty_f32", "dirty2ms_f32", "curig_ms2dirty_dev", "curig_dirty2ms_dev",
     "curig_ms2dirty_multi_dev", "curig_dirty2ms_multi_dev", "curig_explicit_ms2dirty_dev",
-    "curig_nifty_ms2dirty", "curig_nifty_dirty2ms",
+    "curig_nifty_ms2dirty", "curig_nifty_dirty2ms", "curig_nufft_dev",
     "curig_last_error", "curig_plan_info", "curig_nifty_plan_info", "curig_kernel_fseries", "curig_gauss_legendre",
     "curig_set_tuning", "curig_set_fft_path", "curig_set_graph_mode", "curig_enable_stats", "curig_get_stats", "curig_release",
 ]
-# C++-linkage plan API (include/cugridder.h), exported with mangled names
-CXX_EXPORTS = ["gridder_setting", "gridder_execution", "gridder_destroy"]
+# C++-linkage plan APIs (include/cugridder.h, include/cufft_nufft.h), exported with mangled names
+CXX_EXPORTS = ["gridder_setting", "gridder_execution", "gridder_destroy", "cunufft_setting", "cunufft_exec",
+               "curafft_free"]
 
 _lib = None
 
@@ -90,6 +91,9 @@ def load():
                                           c_int, c_double, c_double, c_int,
                                           np.ctypeslib.ndpointer(np.int32, flags="C"), _f64]
     lib.curig_nifty_plan_info.restype = c_int
+    lib.curig_nufft_dev.argtypes = [c_int, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_int, c_double, c_double, c_int, c_void_p]
+    lib.curig_nufft_dev.restype = c_int
     lib.curig_last_error.argtypes = []
     lib.curig_last_error.restype = ctypes.c_char_p
     lib.curig_plan_info.argtypes = [c_int, c_int, c_int, c_double, c_double, c_double, c_double, c_int, c_double,

@@ -2,12 +2,14 @@
 // its C++ plan API, and the additive device-resident / FP32 / multi-channel exports.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <vector>
 
+#include "../../include/cufft_nufft.h"
 #include "../../include/cugridder.h"
 #include "../../include/curig_ext.h"
 #include "curig_internal.h"
@@ -346,6 +348,139 @@ extern "C" int curig_nifty_dirty2ms(int nrow, int nchan, const double *uvw, cons
                     epsilon, do_wstacking, const_cast<void *>(dirty), precision);
 }
 
+// ---------------------------------------------------------------------------
+// Generic NUFFT, types 1 and 2 in 1-3 dimensions, over the same kernels: the
+// one-plane 2-D form with coordinates in radians (the grid phases themselves).
+//   type 1: fk[k3][k2][k1] = sum_j c_j e^{i iflag (k1' u_j + k2' v_j + k3' w_j)}
+//   type 2: c_j = sum_k fk[k3][k2][k1] e^{i iflag (k1' u_j + k2' v_j + k3' w_j)}
+// with k' = k - N/2 (centred modes, N1 the fast index), as the reference's
+// unit drivers define them (test/nufft_2d_1_test.cu, nufft_2d_2_test.cu,
+// nufft_1d_test.cu ground truths).  Dimension 1 runs as N2 = 1; dimension 3
+// as N3 passes of the 2-D transform over c_j e^{i iflag k3' w_j}.
+// ---------------------------------------------------------------------------
+__global__ void pack_nufft_kernel(const double *u, const double *v, double *uvw, int n, double s) {
+  for (long j = blockIdx.x * (long)blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) {
+    uvw[3 * j] = v ? s * v[j] : 0.0;  // slow image axis (N2) pairs with v
+    uvw[3 * j + 1] = s * u[j];       // fast image axis (N1) pairs with u
+    uvw[3 * j + 2] = 0.0;
+  }
+}
+
+template <typename T> struct Cx;
+template <> struct Cx<double> { using type = double2; };
+template <> struct Cx<float> { using type = float2; };
+
+// out_j = c_j e^{i t w_j}
+template <typename T>
+__global__ void phase_mul_kernel(const typename Cx<T>::type *c, const double *w, typename Cx<T>::type *out, int n,
+                                 double t) {
+  for (long j = blockIdx.x * (long)blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) {
+    double sn, cs;
+    sincos(t * w[j], &sn, &cs);
+    const double a = c[j].x, b = c[j].y;
+    out[j].x = (T)(a * cs - b * sn);
+    out[j].y = (T)(a * sn + b * cs);
+  }
+}
+
+// acc_j (+)= x_j e^{i t w_j}
+template <typename T>
+__global__ void phase_acc_kernel(typename Cx<T>::type *acc, const typename Cx<T>::type *x, const double *w, int n,
+                                 double t, int first) {
+  for (long j = blockIdx.x * (long)blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) {
+    double sn, cs;
+    sincos(t * w[j], &sn, &cs);
+    const double a = x[j].x, b = x[j].y;
+    double re = a * cs - b * sn, im = a * sn + b * cs;
+    if (!first) {
+      re += acc[j].x;
+      im += acc[j].y;
+    }
+    acc[j].x = (T)re;
+    acc[j].y = (T)im;
+  }
+}
+
+template <typename T>
+static int nufft_run(int type, int dim, int N1, int N2, int N3, int M, const double *d_u, const double *d_v,
+                     const double *d_w, void *d_c, void *d_fk, int iflag, double eps, double sigma, hipStream_t st) {
+  using C2 = typename Cx<T>::type;
+  const bool dbl = sizeof(T) == 8;
+  const int n2 = dim >= 2 ? N2 : 1, n3 = dim == 3 ? N3 : 1;
+  double *d_uvw = nullptr;
+  C2 *tmp = nullptr;
+  int ier = kOk;
+  const int nb = std::max(1, std::min((M + 255) / 256, 65536));
+  if (M > 0) {
+    CURIG_HIP(hipMalloc(&d_uvw, sizeof(double) * 3 * (size_t)M));
+    // type 1 runs ms2dirty (e^{+i x.k}); type 2 runs dirty2ms (e^{-i x.k})
+    pack_nufft_kernel<<<nb, 256, 0, st>>>(d_u, dim >= 2 ? d_v : nullptr, d_uvw, M, type == 1 ? iflag : -iflag);
+    ier = hipGetLastError() == hipSuccess ? kOk : kErrHip;
+    if (ier == kOk && dim == 3 && hipMalloc(&tmp, sizeof(C2) * (size_t)M) != hipSuccess) ier = kErrAlloc;
+  }
+  Problem pb;
+  pb.nrow = M;
+  pb.nx = n2;
+  pb.ny = N1;
+  pb.freq = 1.0;
+  pb.eps = eps;
+  pb.sigma = sigma;
+  pb.sign = -1;
+  pb.dbl = dbl;
+  pb.uvw = d_uvw;
+  pb.spec.radians = true;
+  pb.spec.wstack = false;
+  const size_t slab = (size_t)N1 * n2;
+  for (int k3 = 0; k3 < n3 && ier == kOk; ++k3) {
+    const double t = (double)iflag * (double)(k3 - n3 / 2);
+    pb.dirty = (C2 *)d_fk + slab * k3;
+    if (type == 1) {
+      pb.vis = d_c;
+      if (dim == 3 && M > 0) {
+        phase_mul_kernel<T><<<nb, 256, 0, st>>>((const C2 *)d_c, d_w, tmp, M, t);
+        pb.vis = tmp;
+      }
+      ier = ms2dirty_device(pb, st, &g_tuning, stats_ptr());
+    } else {
+      pb.vis = dim == 3 ? (void *)tmp : d_c;
+      ier = dirty2ms_device(pb, st, &g_tuning, stats_ptr());
+      if (ier == kOk && dim == 3 && M > 0) phase_acc_kernel<T><<<nb, 256, 0, st>>>((C2 *)d_c, tmp, d_w, M, t, k3 == 0);
+    }
+  }
+  if (ier == kOk && hipStreamSynchronize(st) != hipSuccess) ier = kErrHip;
+  if (d_uvw) (void)hipFree(d_uvw);
+  if (tmp) (void)hipFree(tmp);
+  return ier;
+}
+
+extern "C" int curig_nufft_dev(int type, int dim, int N1, int N2, int N3, int M, const double *d_u,
+                               const double *d_v, const double *d_w, void *d_c, void *d_fk, int iflag,
+                               double epsilon, double sigma, int precision, void *stream) {
+  CURIG_TRY(check_precision(precision));
+  KernelOpts ko;
+  CURIG_TRY(setup_conv_opts(ko, epsilon, sigma));
+  if ((type != 1 && type != 2) || dim < 1 || dim > 3 || (iflag != 1 && iflag != -1)) {
+    set_error("NUFFT type %d, dimension %d, iflag %d: types 1 and 2, dimensions 1-3, iflag +-1", type, dim, iflag);
+    return kErrArg;
+  }
+  if (M < 0) {
+    set_error("M %d < 0", M);
+    return kErrNrow;
+  }
+  if (N1 <= 0 || (dim >= 2 && N2 <= 0) || (dim == 3 && N3 <= 0)) {
+    set_error("mode counts %d x %d x %d invalid", N1, N2, N3);
+    return kErrSize;
+  }
+  if (!d_fk || (M > 0 && (!d_c || !d_u || (dim >= 2 && !d_v) || (dim == 3 && !d_w)))) {
+    set_error("NULL data pointer");
+    return kErrArg;
+  }
+  return precision == 0 ? nufft_run<double>(type, dim, N1, N2, N3, M, d_u, d_v, d_w, d_c, d_fk, iflag, epsilon,
+                                            sigma, (hipStream_t)stream)
+                        : nufft_run<float>(type, dim, N1, N2, N3, M, d_u, d_v, d_w, d_c, d_fk, iflag, epsilon,
+                                           sigma, (hipStream_t)stream);
+}
+
 extern "C" int curig_explicit_ms2dirty_dev(int nrow, int nxdirty, int nydirty, double fov, double freq,
                                            const double *d_uvw, const void *d_vis, const void *d_wgt, int precision,
                                            long npix, const int64_t *d_pix, void *d_out, int sign, void *stream) {
@@ -643,5 +778,85 @@ int gridder_destroy(curafft_plan *plan, ragridder_plan *gridder_plan) {
     free(gridder_plan->kv.weight);
     free(gridder_plan);
   }
+  return kOk;
+}
+
+// ---------------------------------------------------------------------------
+// C++ generic-NUFFT plan API (src/fourier/cuft.cu:249-444)
+// ---------------------------------------------------------------------------
+int cunufft_setting(int N1, int N2, int N3, int M, int kerevalmeth, int method, int direction, PCS tol, PCS sigma,
+                    int type, int dim, PCS *d_u, PCS *d_v, PCS *d_w, CUCPX *d_c, curafft_plan *plan) {
+  if (!plan) {
+    set_error("NULL plan");
+    return kErrArg;
+  }
+  KernelOpts ko;
+  CURIG_TRY(setup_conv_opts(ko, tol, sigma));
+  if ((type != 1 && type != 2) || dim < 1 || dim > 3) {
+    set_error("NUFFT type %d, dimension %d: this library provides types 1 and 2 in dimensions 1-3", type, dim);
+    return kErrArg;
+  }
+  if (M < 0 || N1 <= 0 || (dim >= 2 && N2 <= 0) || (dim == 3 && N3 <= 0)) {
+    set_error("M %d, modes %d x %d x %d invalid", M, N1, N2, N3);
+    return M < 0 ? kErrNrow : kErrSize;
+  }
+  plan->opts.upsampfac = sigma;
+  plan->opts.gpu_sort = 1;
+  plan->opts.gpu_kerevalmeth = kerevalmeth;
+  plan->opts.gpu_gridder_method = method;
+  plan->copts.kw = ko.kw;
+  plan->copts.direction = direction;
+  plan->copts.pirange = 1;
+  plan->copts.upsampfac = sigma;
+  plan->copts.ES_beta = ko.beta;
+  plan->copts.ES_halfwidth = ko.halfwidth;
+  plan->copts.ES_c = ko.c;
+  plan->type = type;
+  plan->dim = dim;
+  plan->mode_flag = 1;
+  plan->iflag = direction > 0 ? 1 : -1;  // cuft.cu:284-285
+  plan->ms = N1;
+  plan->mt = dim >= 2 ? N2 : 1;
+  plan->mu = dim == 3 ? N3 : 1;
+  plan->M = M;
+  plan->d_u = d_u;
+  plan->d_v = d_v;
+  plan->d_w = d_w;
+  plan->d_c = d_c;
+  plan->tol = tol;
+  plan->batchsize = 1;
+  plan->execute_flow = 1;
+  // grid of the 2-D pass (the third dimension is a sum over its N3 modes)
+  GridSpec sp;
+  sp.radians = true;
+  sp.wstack = false;
+  Geom g;
+  CURIG_TRY(plan_geometry(M, plan->mt, N1, 0.0, 1.0, tol, sigma, -1, 0.0, 0.0, plan->opts.precision == 0, &g_tuning,
+                          &g, nullptr, &sp));
+  plan->nf1 = g.nf1;
+  plan->nf2 = g.nf2;
+  plan->nf3 = 1;
+  return kOk;
+}
+
+int cunufft_exec(curafft_plan *plan) {
+  if (!plan || !plan->fk) {
+    set_error("plan->fk (the mode array) must be set");
+    return kErrArg;
+  }
+  return curig_nufft_dev(plan->type, plan->dim, plan->ms, plan->mt, plan->mu, plan->M, plan->d_u, plan->d_v,
+                         plan->d_w, plan->d_c, plan->fk, plan->iflag, plan->tol, plan->copts.upsampfac,
+                         plan->opts.precision, plan->streams ? plan->streams[0] : nullptr);
+}
+
+int curafft_free(curafft_plan *plan) {
+  if (!plan) return kOk;
+  // the plan owns the arrays given to cunufft_setting and plan->fk (cuft.cu:446-478)
+  PCS **own[] = {&plan->d_u, &plan->d_v, &plan->d_w};
+  for (PCS **p : own)
+    if (*p) (void)hipFree(*p), *p = nullptr;
+  if (plan->d_c) (void)hipFree(plan->d_c), plan->d_c = nullptr;
+  if (plan->fw) (void)hipFree(plan->fw), plan->fw = nullptr;
+  if (!plan->opts.gpu_conv_only && plan->fk) (void)hipFree(plan->fk), plan->fk = nullptr;
   return kOk;
 }

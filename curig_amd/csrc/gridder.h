@@ -13,6 +13,8 @@ struct GridSpec {
   double dx = 0, dy = 0;  // pixel sizes in radians along x (u) and y (v); 0: fov / nx, fov / ny
   int nu = 0, nv = 0;     // grid size along u and v (even, > image size); 0: from sigma
   bool wstack = true;     // false: w taken as 0 and the image not divided by n
+  bool radians = false;   // coordinates are the transform's phases per image index (u, v in
+                          // radians; no pixel geometry, freq unused): the plain 2-D NUFFT
 };
 
 struct Problem {

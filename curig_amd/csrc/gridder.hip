@@ -1636,6 +1636,10 @@ struct Timer {
 // Pixel sizes (rad): dx = fov / nx pairs with u, dy = fov / ny with v
 // (src/astro/cugridder.cu:48-49), unless the caller gives both.
 void pixel_sizes(int nx, int ny, double fov, const GridSpec *spec, double &dx, double &dy) {
+  if (spec && spec->radians) {  // no sky geometry: l = m = 0, n = 1
+    dx = dy = 0.0;
+    return;
+  }
   if (spec && (spec->dx != 0 || spec->dy != 0)) {
     dx = spec->dx;
     dy = spec->dy;
@@ -1663,7 +1667,8 @@ int check_args(const Problem &pb) {
     set_error("pixel sizes (%g, %g) must be positive and finite", pb.spec.dx, pb.spec.dy);
     return kErrNonFinite;
   }
-  if ((!own_pix && !(std::isfinite(pb.fov) && pb.fov > 0)) || !(std::isfinite(pb.freq) && pb.freq > 0)) {
+  if (!pb.spec.radians &&
+      ((!own_pix && !(std::isfinite(pb.fov) && pb.fov > 0)) || !(std::isfinite(pb.freq) && pb.freq > 0))) {
     set_error("fov (%g) and freq (%g) must be positive and finite", pb.fov, pb.freq);
     return kErrNonFinite;
   }
@@ -1708,8 +1713,8 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
   // src/astro/cugridder.cu:48-49 (N2 = nx pairs with u, N1 = ny with v)
   pixel_sizes(nx, ny, fov, &sp, g.dx, g.dy);
   double fc = freq / kSpeedOfLight;
-  g.uscale = fc * 2 * M_PI * g.dx;
-  g.vscale = fc * 2 * M_PI * g.dy;
+  g.uscale = sp.radians ? 1.0 : fc * 2 * M_PI * g.dx;
+  g.vscale = sp.radians ? 1.0 : fc * 2 * M_PI * g.dy;
   // 2-D form: every w is taken as 0 (w range 0, C = 0) and the 1 / n dropped
   g.divn = sp.wstack ? 1 : 0;
   g.wscale = sp.wstack ? fc * sign * 2 * M_PI : 0.0;

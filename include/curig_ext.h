@@ -78,6 +78,16 @@ int curig_explicit_ms2dirty_dev(int nrow, int nxdirty, int nydirty, double fov, 
                                 const double *d_uvw, const void *d_vis, const void *d_wgt, int precision,
                                 long npix, const int64_t *d_pix, void *d_out, int sign, void *stream);
 
+/* Generic NUFFT on device pointers (the work behind cunufft_exec, declared in
+ * cufft_nufft.h): type 1 (c -> fk) or 2 (fk -> c), dimension 1-3, modes
+ * N1 (fast) x N2 x N3 (unused dimensions ignored), M points with coordinates
+ * d_u, d_v, d_w (radians; d_v, d_w only as the dimension needs), iflag +-1,
+ * fk [N3][N2][N1] and c [M] complex (precision 0: complex128, 1: complex64).
+ * Synchronous on `stream`. */
+int curig_nufft_dev(int type, int dim, int N1, int N2, int N3, int M, const double *d_u, const double *d_v,
+                    const double *d_w, void *d_c, void *d_fk, int iflag, double epsilon, double sigma, int precision,
+                    void *stream);
+
 /* Text of the last error on the calling thread. */
 const char *curig_last_error(void);
 

@@ -354,3 +354,32 @@ def dirty2ms(uvw, dirty, wgt, nx, ny, fov, freq, eps, sigma=2.0, sign=-1):
         out = out * np.asarray(wgt, dtype=np.float64).ravel()
     out = out * np.exp(1j * ((wp * gamma + C) * D * -1))
     return out
+
+
+# ----------------------------------------------------------------------------
+# generic NUFFT types 1 and 2 (direct sums; small sizes)
+# ----------------------------------------------------------------------------
+def nufft_direct(type_, N, coords, data, iflag):
+    """Direct sums of the generic NUFFT, as the reference's unit drivers define
+    their ground truth (test/nufft_1d_test.cu:289-296, test/nufft_2d_1_test.cu
+    crude direct sum, test/nufft_2d_2_test.cu:319-328):
+
+      type 1: fk[k3][k2][k1] = sum_j c_j e^{i iflag (k1' u_j + k2' v_j + k3' w_j)}
+      type 2: c_j = sum_k fk[k3][k2][k1] e^{i iflag (k1' u_j + k2' v_j + k3' w_j)}
+
+    with k' = k - N/2 (integer halving), N = (N1, [N2, [N3]]) and coords =
+    (u, [v, [w]]) in radians.  fk has shape N[::-1] (N1 the fast index)."""
+    N = tuple(int(n) for n in N)
+    dim = len(N)
+    assert dim == len(coords) and 1 <= dim <= 3
+    E = []
+    for n, x in zip(N, coords):
+        k = np.arange(n) - n // 2
+        E.append(np.exp(1j * iflag * np.outer(np.asarray(x, dtype=np.float64), k)))
+    spec = {1: ("ja->a", "a,ja->j"), 2: ("ja,jb->ba", "ba,ja,jb->j"),
+            3: ("ja,jb,jc->cba", "cba,ja,jb,jc->j")}[dim]
+    if type_ == 1:
+        c = np.asarray(data, dtype=np.complex128)
+        return np.einsum("j," + spec[0], c, *E)
+    fk = np.asarray(data, dtype=np.complex128).reshape(N[::-1])
+    return np.einsum(spec[1], fk, *E)

@@ -481,6 +481,85 @@ extern "C" int curig_nufft_dev(int type, int dim, int N1, int N2, int N3, int M,
                                            sigma, (hipStream_t)stream);
 }
 
+// Type 3 in one dimension, inputs x_j (M) and outputs s_i (N), through the w
+// axis of the same kernels: rows (0, 0, +-x_j) and a 1 x N image whose pixels
+// carry the outputs as their n - 1 (cuft.cu:313-350 type-3 setting; the
+// reference's drivers test/nufft_1d_3_1_test.cu, nufft_1d_3_2_test.cu):
+//   direction 1: f_i = sum_j c_j e^{i iflag s_i x_j}
+//   direction 0: c_j = sum_i f_i e^{i iflag s_i x_j}
+__global__ void pack_type3_kernel(const double *x, double *uvw, int n, double s) {
+  for (long j = blockIdx.x * (long)blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) {
+    uvw[3 * j] = 0.0;
+    uvw[3 * j + 1] = 0.0;
+    uvw[3 * j + 2] = s * x[j];
+  }
+}
+
+extern "C" int curig_nufft3_dev(int M, int N, const double *d_x, const double *d_s, void *d_c, void *d_f, int iflag,
+                                int direction, double epsilon, double sigma, int precision, void *stream) {
+  CURIG_TRY(check_precision(precision));
+  KernelOpts ko;
+  CURIG_TRY(setup_conv_opts(ko, epsilon, sigma));
+  if (iflag != 1 && iflag != -1) {
+    set_error("iflag %d: +-1", iflag);
+    return kErrArg;
+  }
+  if (M < 0) {
+    set_error("M %d < 0", M);
+    return kErrNrow;
+  }
+  if (N <= 0) {
+    set_error("N %d: at least one output", N);
+    return kErrSize;
+  }
+  if (!d_s || !d_f || (M > 0 && (!d_x || !d_c))) {
+    set_error("NULL data pointer");
+    return kErrArg;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  // the outputs' range sets the plan (one wait)
+  std::vector<double> hs(N);
+  CURIG_HIP(hipMemcpyAsync(hs.data(), d_s, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+  CURIG_HIP(hipStreamSynchronize(st));
+  double smin = hs[0], smax = hs[0];
+  for (double v : hs) {
+    if (!std::isfinite(v)) {
+      set_error("non-finite output coordinate");
+      return kErrNonFinite;
+    }
+    smin = std::min(smin, v);
+    smax = std::max(smax, v);
+  }
+  double *d_uvw = nullptr;
+  if (M > 0) {
+    CURIG_HIP(hipMalloc(&d_uvw, sizeof(double) * 3 * (size_t)M));
+    // direction 1 runs ms2dirty (e^{+i w z}), direction 0 dirty2ms (e^{-i w z})
+    const int nb = std::max(1, std::min((M + 255) / 256, 65536));
+    pack_type3_kernel<<<nb, 256, 0, st>>>(d_x, d_uvw, M, direction == 1 ? iflag : -iflag);
+  }
+  Problem pb;
+  pb.nrow = M;
+  pb.nx = 1;
+  pb.ny = N;
+  pb.freq = 1.0;
+  pb.eps = epsilon;
+  pb.sigma = sigma;
+  pb.sign = 1;
+  pb.dbl = precision == 0;
+  pb.uvw = d_uvw;
+  pb.vis = d_c;
+  pb.dirty = d_f;
+  pb.spec.radians = true;
+  pb.spec.zout = d_s;
+  pb.spec.zmin = smin;
+  pb.spec.zmax = smax;
+  int ier = direction == 1 ? ms2dirty_device(pb, st, &g_tuning, stats_ptr())
+                           : dirty2ms_device(pb, st, &g_tuning, stats_ptr());
+  if (ier == kOk && hipStreamSynchronize(st) != hipSuccess) ier = kErrHip;
+  if (d_uvw) (void)hipFree(d_uvw);
+  return ier;
+}
+
 extern "C" int curig_explicit_ms2dirty_dev(int nrow, int nxdirty, int nydirty, double fov, double freq,
                                            const double *d_uvw, const void *d_vis, const void *d_wgt, int precision,
                                            long npix, const int64_t *d_pix, void *d_out, int sign, void *stream) {
@@ -792,8 +871,8 @@ int cunufft_setting(int N1, int N2, int N3, int M, int kerevalmeth, int method, 
   }
   KernelOpts ko;
   CURIG_TRY(setup_conv_opts(ko, tol, sigma));
-  if ((type != 1 && type != 2) || dim < 1 || dim > 3) {
-    set_error("NUFFT type %d, dimension %d: this library provides types 1 and 2 in dimensions 1-3", type, dim);
+  if (!((type == 1 || type == 2) && dim >= 1 && dim <= 3) && !(type == 3 && dim == 1)) {
+    set_error("NUFFT type %d, dimension %d: types 1 and 2 in dimensions 1-3, type 3 in dimension 1", type, dim);
     return kErrArg;
   }
   if (M < 0 || N1 <= 0 || (dim >= 2 && N2 <= 0) || (dim == 3 && N3 <= 0)) {
@@ -826,6 +905,16 @@ int cunufft_setting(int N1, int N2, int N3, int M, int kerevalmeth, int method, 
   plan->tol = tol;
   plan->batchsize = 1;
   plan->execute_flow = 1;
+  if (type == 3) {
+    // type 3: inputs d_u (M), outputs plan->d_x (N1), set by the caller before
+    // this call (cuft.cu:313-330); the grid follows from both ranges at exec
+    if (!plan->d_x || (M > 0 && !d_u)) {
+      set_error("type 3 needs plan->d_x (the N1 output coordinates) and d_u");
+      return kErrArg;
+    }
+    plan->nf1 = plan->nf2 = plan->nf3 = 0;
+    return kOk;
+  }
   // grid of the 2-D pass (the third dimension is a sum over its N3 modes)
   GridSpec sp;
   sp.radians = true;
@@ -844,6 +933,10 @@ int cunufft_exec(curafft_plan *plan) {
     set_error("plan->fk (the mode array) must be set");
     return kErrArg;
   }
+  if (plan->type == 3)
+    return curig_nufft3_dev(plan->M, plan->ms, plan->d_u, plan->d_x, plan->d_c, plan->fk, plan->iflag,
+                            plan->copts.direction > 0 ? 1 : 0, plan->tol, plan->copts.upsampfac, plan->opts.precision,
+                            plan->streams ? plan->streams[0] : nullptr);
   return curig_nufft_dev(plan->type, plan->dim, plan->ms, plan->mt, plan->mu, plan->M, plan->d_u, plan->d_v,
                          plan->d_w, plan->d_c, plan->fk, plan->iflag, plan->tol, plan->copts.upsampfac,
                          plan->opts.precision, plan->streams ? plan->streams[0] : nullptr);

@@ -79,6 +79,7 @@ struct Geom {
   double beta, c;     // normalised ES kernel exp(beta (sqrt(1 - c d^2) - 1))
   double dx, dy;      // pixel sizes (rad): dx pairs with u / image rows, dy with v / columns
   int divn;           // 1: image = sum / n (w-stacking); 0: w taken as 0 and no 1/n (2-D transform)
+  const double *zout; // device [N2][N1]: each pixel's n - 1 given (type-3 outputs; n taken as 1), or null
   double C, D, gamma, h;  // type-3 w parameters (t3attrib): i_center, o_center, gamma, h
   double uscale, vscale, wscale;  // 2 pi f/c dx, 2 pi f/c dy, 2 pi f/c sign
   int T1, T2, ntx, nty, ntiles;   // tile size (cells) along nf1 / nf2, tile counts

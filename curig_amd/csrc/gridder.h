@@ -15,6 +15,10 @@ struct GridSpec {
   bool wstack = true;     // false: w taken as 0 and the image not divided by n
   bool radians = false;   // coordinates are the transform's phases per image index (u, v in
                           // radians; no pixel geometry, freq unused): the plain 2-D NUFFT
+  // with radians and wstack: w in radians as well, and each pixel's n - 1 read from
+  // zout (device, [nx][ny]) whose values span [zmin, zmax] (a 1-D type-3 transform)
+  const double *zout = nullptr;
+  double zmin = 0, zmax = 0;
 };
 
 struct Problem {

@@ -88,6 +88,11 @@ __device__ __forceinline__ double rescale_fold(double x, int nf) {
 
 // n - 1 without cancellation, and n, for pixel (row, col)
 __device__ __forceinline__ void pixel_n(int row, int col, const Geom &g, double &nm1, double &n) {
+  if (g.zout) {  // outputs of a type-3 transform: n - 1 given per pixel
+    nm1 = g.zout[(long)row * g.N1 + col];
+    n = 1.0;
+    return;
+  }
   double l = (double)(row - g.N2 / 2) * g.dx;
   double m = (double)(col - g.N1 / 2) * g.dy;
   double r2 = l * l + m * m;
@@ -1717,7 +1722,8 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
   g.vscale = sp.radians ? 1.0 : fc * 2 * M_PI * g.dy;
   // 2-D form: every w is taken as 0 (w range 0, C = 0) and the 1 / n dropped
   g.divn = sp.wstack ? 1 : 0;
-  g.wscale = sp.wstack ? fc * sign * 2 * M_PI : 0.0;
+  g.wscale = sp.wstack ? (sp.radians ? 1.0 : fc * sign * 2 * M_PI) : 0.0;
+  g.zout = sp.zout;
   // type-3 parameters (cugridder.cu:52-76)
   double l_min = -0.5 * g.dx * nx, m_min = -0.5 * g.dy * ny;
   double n_lm = std::sqrt(1.0 - l_min * l_min - m_min * m_min);
@@ -1729,6 +1735,10 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
   double o_min = n_lm - 1;
   g.D = o_min / 2.0;
   double S = std::fabs(o_min / 2.0);
+  if (sp.zout) {  // given outputs: centre and half-width of their range
+    g.D = 0.5 * (sp.zmin + sp.zmax);
+    S = 0.5 * (sp.zmax - sp.zmin);
+  }
   set_nhg_type3(S, X, ko, g.P, g.h, g.gamma);
   g.kww = g.kw;
   if (!sp.wstack) {

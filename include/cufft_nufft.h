@@ -2,9 +2,11 @@
  * include/cuft.h:7-8 (cunufft_setting) and src/fourier/cuft.cu:426-478
  * (cunufft_exec, curafft_free) with the same names and arguments.
  *
- * Types 1 and 2 in 1-3 dimensions (the reference's type 3 is out of scope here):
+ * Types 1 and 2 in 1-3 dimensions and type 3 in one dimension, as the reference:
  *   type 1: fk[k3][k2][k1] = sum_j c_j e^{i iflag (k1' u_j + k2' v_j + k3' w_j)}
  *   type 2: c_j = sum_k fk[k3][k2][k1] e^{i iflag (k1' u_j + k2' v_j + k3' w_j)}
+ *   type 3: outputs s_i = plan->d_x[i] (N1 of them, set before cunufft_setting):
+ *           direction 1: fk_i = sum_j c_j e^{i s_i u_j}; direction 0: c_j = sum_i fk_i e^{-i s_i u_j}
  * k' = k - N/2 (centred modes), N1 the fast index, u, v, w in radians (any
  * real value: the transform is 2 pi periodic), iflag = +1 for direction > 0,
  * else -1 (src/fourier/cuft.cu:284-285).  plan->fk is set by the caller before

@@ -88,6 +88,15 @@ int curig_nufft_dev(int type, int dim, int N1, int N2, int N3, int M, const doub
                     const double *d_w, void *d_c, void *d_fk, int iflag, double epsilon, double sigma, int precision,
                     void *stream);
 
+/* Type-3 NUFFT in one dimension on device pointers (the work behind
+ * cunufft_exec for type 3): inputs x [M], outputs s [N] (radians, FP64),
+ *   direction 1: f_i = sum_j c_j e^{i iflag s_i x_j}   (c -> f)
+ *   direction 0: c_j = sum_i f_i e^{i iflag s_i x_j}   (f -> c)
+ * c [M], f [N] complex (precision 0: complex128, 1: complex64).  Computed on
+ * the w axis of the gridder (one wait for the range of s).  Synchronous. */
+int curig_nufft3_dev(int M, int N, const double *d_x, const double *d_s, void *d_c, void *d_f, int iflag,
+                     int direction, double epsilon, double sigma, int precision, void *stream);
+
 /* Text of the last error on the calling thread. */
 const char *curig_last_error(void);
 

@@ -383,3 +383,16 @@ def nufft_direct(type_, N, coords, data, iflag):
         return np.einsum("j," + spec[0], c, *E)
     fk = np.asarray(data, dtype=np.complex128).reshape(N[::-1])
     return np.einsum(spec[1], fk, *E)
+
+
+def nufft3_direct(x, s, data, iflag, direction):
+    """Direct sums of the 1-D type-3 transform with inputs x (M) and outputs s
+    (N), as the reference's drivers define their ground truth
+    (test/nufft_1d_3_1_test.cu:210-219, direction 1 with iflag +1;
+    test/nufft_1d_3_2_test.cu:180-187, direction 0 with iflag -1):
+
+      direction 1: f_i = sum_j c_j e^{i iflag s_i x_j}   (data = c, returns f)
+      direction 0: c_j = sum_i f_i e^{i iflag s_i x_j}   (data = f, returns c)"""
+    E = np.exp(1j * iflag * np.outer(np.asarray(s, dtype=np.float64), np.asarray(x, dtype=np.float64)))  # [N, M]
+    d = np.asarray(data, dtype=np.complex128)
+    return E @ d if direction == 1 else d @ E

@@ -89,3 +89,69 @@ def test_nufft_adjoint(N):
     lhs, rhs = np.vdot(fk, a), np.vdot(b, c)
     say(f"  rel {abs(lhs - rhs) / abs(lhs):.3e}")
     assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def _run3(M, N, x, s, data, iflag, direction, eps, precision):
+    import torch
+
+    from curig_amd import _lib
+
+    lib = _lib.load()
+    dev = torch.device("cuda:0")
+    cd = torch.complex128 if precision == 0 else torch.complex64
+    dx = torch.tensor(np.asarray(x, dtype=np.float64), device=dev)
+    ds = torch.tensor(np.asarray(s, dtype=np.float64), device=dev)
+    if direction == 1:
+        c = torch.tensor(np.asarray(data), dtype=cd, device=dev)
+        f = torch.zeros(N, dtype=cd, device=dev)
+    else:
+        f = torch.tensor(np.asarray(data), dtype=cd, device=dev)
+        c = torch.zeros(M, dtype=cd, device=dev)
+    code = lib.curig_nufft3_dev(M, N, dx.data_ptr(), ds.data_ptr(), c.data_ptr(), f.data_ptr(), iflag, direction,
+                                eps, 2.0, precision, torch.cuda.current_stream().cuda_stream)
+    assert code == 0, lib.curig_last_error()
+    torch.cuda.synchronize()
+    return (f if direction == 1 else c).cpu().numpy().astype(np.complex128)
+
+
+# (x range, s range): a wide product (hundreds of w planes) and the reference
+# driver's case (x = 2 + j pi / M, s = -i / N)
+T3 = [("wide", 1500, 300), ("reference", 1000, 200)]
+
+
+def _case3(kind, M, N, seed):
+    rng = np.random.default_rng(seed)
+    if kind == "wide":
+        x, s = rng.uniform(-20, 20, M), rng.uniform(-15, 15, N)
+    else:
+        x, s = 2.0 + np.arange(M) * np.pi / M, -np.arange(N) / N
+    return x, s, rng.standard_normal(M) + 1j * rng.standard_normal(M), rng.standard_normal(N) + 1j * rng.standard_normal(N)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("iflag", [1, -1])
+@pytest.mark.parametrize("direction", [1, 0])
+@pytest.mark.parametrize("case", T3, ids=[t[0] for t in T3])
+def test_nufft3_matches_direct_sum(case, direction, iflag, precision):
+    kind, M, N = case
+    eps = 1e-6 if precision == 0 else 1e-5
+    x, s, c, f = _case3(kind, M, N, 21 + direction)
+    say(f"[nufft3] {kind} M {M} N {N} direction {direction} iflag {iflag} precision {precision}")
+    data = c if direction == 1 else f
+    got = _run3(M, N, x, s, data, iflag, direction, eps, precision)
+    want = R.nufft3_direct(x, s, data, iflag, direction)
+    err = _err(want, got)
+    tol = 3 * eps + 1e-11 + (3e-6 if precision else 0)
+    say(f"  err {err:.3e} tol {tol:.1e}")
+    assert err < tol
+
+
+def test_nufft3_adjoint():
+    M, N = 1500, 300
+    x, s, c, f = _case3("wide", M, N, 31)
+    say("[nufft3 adjoint]")
+    a = _run3(M, N, x, s, c, 1, 1, 1e-9, 0)
+    b = _run3(M, N, x, s, f, -1, 0, 1e-9, 0)
+    lhs, rhs = np.vdot(f, a), np.vdot(b, c)
+    say(f"  rel {abs(lhs - rhs) / abs(lhs):.3e}")
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)

@@ -81,3 +81,43 @@ def test_nufft_argument_errors():
     assert call(dim=3, N=(8, 8, 0)) == 3
     assert call(prec=2) == 5
     assert call(fk=None) == 5
+
+
+@pytest.mark.parametrize("iflag", [1, -1])
+def test_type3_on_integer_outputs_is_type1(iflag):
+    # outputs at the centred integer modes turn type 3 into type 1 (and its
+    # adjoint direction into type 2)
+    rng = np.random.default_rng(9)
+    M, N = 30, 11
+    x = rng.uniform(-np.pi, np.pi, M)
+    c = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    f = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    s = np.arange(N) - N // 2
+    np.testing.assert_allclose(R.nufft3_direct(x, s, c, iflag, 1), R.nufft_direct(1, (N,), (x,), c, iflag),
+                               rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(R.nufft3_direct(x, s, f, iflag, 0), R.nufft_direct(2, (N,), (x,), f, iflag),
+                               rtol=1e-12, atol=1e-12)
+
+
+def test_type3_directions_adjoint():
+    rng = np.random.default_rng(10)
+    x, s = rng.uniform(-30, 30, 20), rng.uniform(-5, 5, 13)
+    c = rng.standard_normal(20) + 1j * rng.standard_normal(20)
+    f = rng.standard_normal(13) + 1j * rng.standard_normal(13)
+    a = R.nufft3_direct(x, s, c, 1, 1)
+    b = R.nufft3_direct(x, s, f, -1, 0)
+    assert abs(np.vdot(f, a) - np.vdot(b, c)) <= 1e-12 * abs(np.vdot(f, a))
+
+
+def test_type3_argument_errors():
+    lib = _lib.load()
+
+    def call(M=4, N=5, iflag=1, eps=1e-6, prec=0, s=8, f=8):
+        return lib.curig_nufft3_dev(M, N, 8, s, 8, f, iflag, 1, eps, 2.0, prec, None)
+
+    assert call(eps=1e-17) == 1
+    assert call(iflag=2) == 5
+    assert call(M=-1) == 4
+    assert call(N=0) == 3
+    assert call(s=None) == 5
+    assert call(prec=3) == 5

@@ -349,6 +349,51 @@ def form_nifty(args, log):
     }
 
 
+def form_nufft(args, log):
+    """Generic 2-D NUFFT: one type 1 and one type 2 per step, 4096 x 4096 modes,
+    20M points uniform in [-pi, pi)^2, epsilon 1e-5, FP32-mixed, inputs resident
+    on the GPU (curig_nufft_dev).  Parity is covered by the tests."""
+    import torch
+
+    from curig_amd import _lib
+
+    lib = _lib.load()
+    N, M, eps = 4096, 20_000_000, args.eps or 1e-5
+    rng = np.random.default_rng(5)
+    dev = torch.device("cuda:0")
+    u = torch.from_numpy(rng.uniform(-np.pi, np.pi, M)).to(dev)
+    v = torch.from_numpy(rng.uniform(-np.pi, np.pi, M)).to(dev)
+    c = torch.complex(torch.randn(M, device=dev), torch.randn(M, device=dev))
+    fk = torch.zeros((N, N), dtype=torch.complex64, device=dev)
+    c2 = torch.zeros_like(c)
+    st = torch.cuda.current_stream().cuda_stream
+    log(f"[bench] nufft: 2-D types 1 + 2, {N}x{N} modes, {M} points, eps {eps}, fp32")
+
+    def step():
+        _lib.check(lib.curig_nufft_dev(1, 2, N, N, 1, M, u.data_ptr(), v.data_ptr(), None, c.data_ptr(),
+                                       fk.data_ptr(), 1, eps, 2.0, 1, st))
+        _lib.check(lib.curig_nufft_dev(2, 2, N, N, 1, M, u.data_ptr(), v.data_ptr(), None, c2.data_ptr(),
+                                       fk.data_ptr(), -1, eps, 2.0, 1, st))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / args.steps
+    value = M / t / 1e6
+    log(f"[bench] nufft {t * 1e3:.1f} ms/step ({value:.1f} Mpts/s, type 1 + type 2)")
+    return {
+        "metric": "Mpts/s generic 2-D NUFFT type 1 + type 2", "value": round(value, 3), "unit": "Mpts/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (uniform points, Gaussian strengths, seeded)",
+        "config": {"workload": f"2-D NUFFT {N}x{N} modes, {M} points in [-pi,pi)^2, eps {eps}, fp32"},
+    }
+
+
 def form_batched(args, log):
     """64 channels linspace(1.0, 1.5) GHz of the 128-antenna 5 km array (256
     snapshots over +-3 h), per-channel flag fraction U[0, 0.4] as i.i.d. row
@@ -567,7 +612,7 @@ def main():
     ap.add_argument("--no-err", action="store_true")
     ap.add_argument("--no-api", action="store_true", help="skip the host-pointer (PCIe-inclusive) timing")
     ap.add_argument("--no-eps4", action="store_true", help="skip the epsilon 1e-4 run of the same workload")
-    ap.add_argument("--form", default="precision", choices=["precision", "batched", "diff", "nifty"])
+    ap.add_argument("--form", default="precision", choices=["precision", "batched", "diff", "nifty", "nufft"])
     ap.add_argument("--lr", type=float, default=0.02, help="Adam step of the diff form")
     ap.add_argument("--traffic", type=float, default=None,
                     help="HBM bytes per launch of the dominant kernel from a counter run (reported as given)")
@@ -580,7 +625,8 @@ def main():
         sys.exit(2)
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
     if args.form != "precision":
-        result = {"batched": form_batched, "diff": form_diff, "nifty": form_nifty}[args.form](args, log)
+        result = {"batched": form_batched, "diff": form_diff, "nifty": form_nifty,
+                  "nufft": form_nufft}[args.form](args, log)
         print(json.dumps(result), flush=True)
         return
     cfg = make_config(args.config, args.precision, args.eps)

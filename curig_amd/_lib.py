@@ -23,7 +23,7 @@ EXPORTS = [
     "curig_ms2dirty_multi_dev", "curig_dirty2ms_multi_dev", "curig_explicit_ms2dirty_dev",
     "curig_nifty_ms2dirty", "curig_nifty_dirty2ms", "curig_nufft_dev", "curig_nufft3_dev",
     "curig_last_error", "curig_plan_info", "curig_nifty_plan_info", "curig_kernel_fseries", "curig_gauss_legendre",
-    "curig_set_tuning", "curig_set_fft_path", "curig_set_graph_mode", "curig_enable_stats", "curig_get_stats", "curig_release",
+    "curig_set_tuning", "curig_set_kernel_eval", "curig_kernel_horner", "curig_set_fft_path", "curig_set_graph_mode", "curig_enable_stats", "curig_get_stats", "curig_release",
 ]
 # C++-linkage plan APIs (include/cugridder.h, include/cufft_nufft.h), exported with mangled names
 CXX_EXPORTS = ["gridder_setting", "gridder_execution", "gridder_destroy", "cunufft_setting", "cunufft_exec",
@@ -108,6 +108,11 @@ def load():
     lib.curig_gauss_legendre.restype = c_int
     lib.curig_set_tuning.argtypes = [c_int, c_int, c_int, c_int]
     lib.curig_set_tuning.restype = c_int
+    lib.curig_set_kernel_eval.argtypes = [c_int]
+    lib.curig_set_kernel_eval.restype = c_int
+    lib.curig_kernel_horner.argtypes = [c_double, c_double, c_int, np.ctypeslib.ndpointer(np.int32, flags="C"),
+                                        _f64, c_int]
+    lib.curig_kernel_horner.restype = c_int
     lib.curig_set_fft_path.argtypes = [c_int]
     lib.curig_set_fft_path.restype = c_int
     lib.curig_set_graph_mode.argtypes = [c_int]

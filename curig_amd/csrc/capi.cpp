@@ -403,7 +403,8 @@ __global__ void phase_acc_kernel(typename Cx<T>::type *acc, const typename Cx<T>
 
 template <typename T>
 static int nufft_run(int type, int dim, int N1, int N2, int N3, int M, const double *d_u, const double *d_v,
-                     const double *d_w, void *d_c, void *d_fk, int iflag, double eps, double sigma, hipStream_t st) {
+                     const double *d_w, void *d_c, void *d_fk, int iflag, double eps, double sigma, hipStream_t st,
+                     const Tuning &tn) {
   using C2 = typename Cx<T>::type;
   const bool dbl = sizeof(T) == 8;
   const int n2 = dim >= 2 ? N2 : 1, n3 = dim == 3 ? N3 : 1;
@@ -440,10 +441,10 @@ static int nufft_run(int type, int dim, int N1, int N2, int N3, int M, const dou
         phase_mul_kernel<T><<<nb, 256, 0, st>>>((const C2 *)d_c, d_w, tmp, M, t);
         pb.vis = tmp;
       }
-      ier = ms2dirty_device(pb, st, &g_tuning, stats_ptr());
+      ier = ms2dirty_device(pb, st, &tn, stats_ptr());
     } else {
       pb.vis = dim == 3 ? (void *)tmp : d_c;
-      ier = dirty2ms_device(pb, st, &g_tuning, stats_ptr());
+      ier = dirty2ms_device(pb, st, &tn, stats_ptr());
       if (ier == kOk && dim == 3 && M > 0) phase_acc_kernel<T><<<nb, 256, 0, st>>>((C2 *)d_c, tmp, d_w, M, t, k3 == 0);
     }
   }
@@ -453,9 +454,9 @@ static int nufft_run(int type, int dim, int N1, int N2, int N3, int M, const dou
   return ier;
 }
 
-extern "C" int curig_nufft_dev(int type, int dim, int N1, int N2, int N3, int M, const double *d_u,
-                               const double *d_v, const double *d_w, void *d_c, void *d_fk, int iflag,
-                               double epsilon, double sigma, int precision, void *stream) {
+static int nufft_checked(int type, int dim, int N1, int N2, int N3, int M, const double *d_u, const double *d_v,
+                         const double *d_w, void *d_c, void *d_fk, int iflag, double epsilon, double sigma,
+                         int precision, void *stream, const Tuning &tn) {
   CURIG_TRY(check_precision(precision));
   KernelOpts ko;
   CURIG_TRY(setup_conv_opts(ko, epsilon, sigma));
@@ -476,9 +477,16 @@ extern "C" int curig_nufft_dev(int type, int dim, int N1, int N2, int N3, int M,
     return kErrArg;
   }
   return precision == 0 ? nufft_run<double>(type, dim, N1, N2, N3, M, d_u, d_v, d_w, d_c, d_fk, iflag, epsilon,
-                                            sigma, (hipStream_t)stream)
+                                            sigma, (hipStream_t)stream, tn)
                         : nufft_run<float>(type, dim, N1, N2, N3, M, d_u, d_v, d_w, d_c, d_fk, iflag, epsilon,
-                                           sigma, (hipStream_t)stream);
+                                           sigma, (hipStream_t)stream, tn);
+}
+
+extern "C" int curig_nufft_dev(int type, int dim, int N1, int N2, int N3, int M, const double *d_u,
+                               const double *d_v, const double *d_w, void *d_c, void *d_fk, int iflag,
+                               double epsilon, double sigma, int precision, void *stream) {
+  return nufft_checked(type, dim, N1, N2, N3, M, d_u, d_v, d_w, d_c, d_fk, iflag, epsilon, sigma, precision, stream,
+                       g_tuning);
 }
 
 // Type 3 in one dimension, inputs x_j (M) and outputs s_i (N), through the w
@@ -495,8 +503,8 @@ __global__ void pack_type3_kernel(const double *x, double *uvw, int n, double s)
   }
 }
 
-extern "C" int curig_nufft3_dev(int M, int N, const double *d_x, const double *d_s, void *d_c, void *d_f, int iflag,
-                                int direction, double epsilon, double sigma, int precision, void *stream) {
+static int nufft3_checked(int M, int N, const double *d_x, const double *d_s, void *d_c, void *d_f, int iflag,
+                          int direction, double epsilon, double sigma, int precision, void *stream, const Tuning &tn) {
   CURIG_TRY(check_precision(precision));
   KernelOpts ko;
   CURIG_TRY(setup_conv_opts(ko, epsilon, sigma));
@@ -553,11 +561,15 @@ extern "C" int curig_nufft3_dev(int M, int N, const double *d_x, const double *d
   pb.spec.zout = d_s;
   pb.spec.zmin = smin;
   pb.spec.zmax = smax;
-  int ier = direction == 1 ? ms2dirty_device(pb, st, &g_tuning, stats_ptr())
-                           : dirty2ms_device(pb, st, &g_tuning, stats_ptr());
+  int ier = direction == 1 ? ms2dirty_device(pb, st, &tn, stats_ptr()) : dirty2ms_device(pb, st, &tn, stats_ptr());
   if (ier == kOk && hipStreamSynchronize(st) != hipSuccess) ier = kErrHip;
   if (d_uvw) (void)hipFree(d_uvw);
   return ier;
+}
+
+extern "C" int curig_nufft3_dev(int M, int N, const double *d_x, const double *d_s, void *d_c, void *d_f, int iflag,
+                                int direction, double epsilon, double sigma, int precision, void *stream) {
+  return nufft3_checked(M, N, d_x, d_s, d_c, d_f, iflag, direction, epsilon, sigma, precision, stream, g_tuning);
 }
 
 extern "C" int curig_explicit_ms2dirty_dev(int nrow, int nxdirty, int nydirty, double fov, double freq,
@@ -626,6 +638,37 @@ extern "C" int curig_nifty_plan_info(int nrow_valid, int nxdirty, int nydirty, d
   if (reals) {
     double r[8] = {ko.beta, ko.c, g.gamma, g.h, g.C, g.D, g.dx, g.dy};
     memcpy(reals, r, sizeof(r));
+  }
+  return kOk;
+}
+
+extern "C" int curig_set_kernel_eval(int method) {
+  if (method != 0 && method != 1) {
+    set_error("kernel evaluation method %d: 0 (exp of sqrt) or 1 (piecewise polynomial)", method);
+    return kErrArg;
+  }
+  g_tuning.kereval = method;
+  return kOk;
+}
+
+extern "C" int curig_kernel_horner(double epsilon, double sigma, int precision, int *deg, double *coef,
+                                   int max_coef) {
+  CURIG_TRY(check_precision(precision));
+  KernelOpts ko;
+  CURIG_TRY(setup_conv_opts(ko, epsilon, sigma));
+  std::vector<double> c;
+  const int d = horner_fit(ko, horner_tol(epsilon, precision == 0), c);
+  if (d < 0) {
+    set_error("no polynomial of degree <= %d fits the kernel", kMaxHornerDeg);
+    return kErrArg;
+  }
+  if (deg) *deg = d;
+  if (coef) {
+    if ((int)c.size() > max_coef) {
+      set_error("%zu coefficients, room for %d", c.size(), max_coef);
+      return kErrArg;
+    }
+    memcpy(coef, c.data(), sizeof(double) * c.size());
   }
   return kOk;
 }
@@ -829,8 +872,10 @@ int gridder_execution(curafft_plan *plan, ragridder_plan *gridder_plan) {
   pb.dirty = plan->fk;
   pb.eps = plan->tol;
   hipStream_t st = plan->streams ? plan->streams[0] : nullptr;
-  int ier = plan->copts.direction == 1 ? ms2dirty_device(pb, st, &g_tuning, stats_ptr())
-                                       : dirty2ms_device(pb, st, &g_tuning, stats_ptr());
+  Tuning tn = g_tuning;
+  tn.kereval = plan->opts.gpu_kerevalmeth == 1 ? 1 : 0;
+  int ier = plan->copts.direction == 1 ? ms2dirty_device(pb, st, &tn, stats_ptr())
+                                       : dirty2ms_device(pb, st, &tn, stats_ptr());
   if (ier != kOk) return ier;
   CURIG_HIP(hipStreamSynchronize(st));
   return kOk;
@@ -933,13 +978,15 @@ int cunufft_exec(curafft_plan *plan) {
     set_error("plan->fk (the mode array) must be set");
     return kErrArg;
   }
+  Tuning tn = g_tuning;
+  tn.kereval = plan->opts.gpu_kerevalmeth == 1 ? 1 : 0;
   if (plan->type == 3)
-    return curig_nufft3_dev(plan->M, plan->ms, plan->d_u, plan->d_x, plan->d_c, plan->fk, plan->iflag,
-                            plan->copts.direction > 0 ? 1 : 0, plan->tol, plan->copts.upsampfac, plan->opts.precision,
-                            plan->streams ? plan->streams[0] : nullptr);
-  return curig_nufft_dev(plan->type, plan->dim, plan->ms, plan->mt, plan->mu, plan->M, plan->d_u, plan->d_v,
-                         plan->d_w, plan->d_c, plan->fk, plan->iflag, plan->tol, plan->copts.upsampfac,
-                         plan->opts.precision, plan->streams ? plan->streams[0] : nullptr);
+    return nufft3_checked(plan->M, plan->ms, plan->d_u, plan->d_x, plan->d_c, plan->fk, plan->iflag,
+                          plan->copts.direction > 0 ? 1 : 0, plan->tol, plan->copts.upsampfac, plan->opts.precision,
+                          plan->streams ? plan->streams[0] : nullptr, tn);
+  return nufft_checked(plan->type, plan->dim, plan->ms, plan->mt, plan->mu, plan->M, plan->d_u, plan->d_v,
+                       plan->d_w, plan->d_c, plan->fk, plan->iflag, plan->tol, plan->copts.upsampfac,
+                       plan->opts.precision, plan->streams ? plan->streams[0] : nullptr, tn);
 }
 
 int curafft_free(curafft_plan *plan) {

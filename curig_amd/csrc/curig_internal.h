@@ -69,6 +69,11 @@ void gauss_legendre(int n, double *x, double *w);
 int kernel_quadrature(const KernelOpts &o, double *x, double *g);
 // phihat_norm(k) (-1)^k for k = 0..nf/2 (normalised kernel, peak 1)
 void fseries_grid(const KernelOpts &o, int nf, double *out);
+// piecewise-polynomial kernel: kw intervals x (deg + 1) coefficients in powers
+// of z in [-1, 1], highest first; returns deg (-1 when tol is not reached)
+constexpr int kMaxHornerDeg = 24;
+int horner_fit(const KernelOpts &o, double tol, std::vector<double> &coef);
+double horner_tol(double eps, bool dbl);
 
 // Everything the device kernels need for one (channel, direction) problem.
 struct Geom {
@@ -76,6 +81,9 @@ struct Geom {
   int nf1, nf2, P;    // grid sizes: nf1 (v), nf2 (u), w planes
   int kw;
   int kww;            // kernel width along w: kw, or 1 in the 2-D form (one plane, weight 1)
+  int hdeg;           // >= 0: kernel values by Horner's rule on the piecewise polynomial below
+  const double *hcoef_d;  // [kw][hdeg + 1] coefficients (device), FP64 and FP32 copies
+  const float *hcoef_f;
   double beta, c;     // normalised ES kernel exp(beta (sqrt(1 - c d^2) - 1))
   double dx, dy;      // pixel sizes (rad): dx pairs with u / image rows, dy with v / columns
   int divn;           // 1: image = sum / n (w-stacking); 0: w taken as 0 and no 1/n (2-D transform)
@@ -120,6 +128,7 @@ struct Tuning {
   int Pb = 0;           // 0 -> from the memory budget
   int fft = 0;          // 0 auto (LDS path when it applies), 1 force rocFFT
   int lds_bytes = 0;    // LDS per workgroup of the device (0: 160 KiB, gfx950)
+  int kereval = 0;      // kernel evaluation: 0 exp(sqrt()), 1 piecewise polynomial (Horner)
   size_t grid_budget_bytes = size_t(48) << 30;
 };
 

@@ -74,6 +74,41 @@ __device__ __forceinline__ double es_eval(double d, double beta, double c, doubl
   return (fabs(d) < hw && t < 1.0) ? exp(-beta * t / (sqrt(1.0 - t) + 1.0)) : 0.0;
 }
 
+// Kernel value at tap i of a record whose first tap lies at offset d (d + i in
+// [-kw/2, kw/2)): exp(sqrt()) evaluation, or Horner's rule on the piecewise
+// polynomial of interval i at z = 2 (d + kw/2) - 1 (hdeg >= 0).
+template <typename T> struct Ker {
+  T beta, c, hw;
+  const T *coef;
+  int deg;
+  bool flatw;  // 2-D form: the single plane's weight is 1
+  // weight of plane tap k (first tap at offset d)
+  __device__ __forceinline__ T w(T d, int k) const { return flatw ? (T)1 : (*this)(d, k); }
+  __device__ __forceinline__ T operator()(T d, int i) const {
+    if (deg >= 0) {
+      const T z = (T)2 * (d + hw) - (T)1;
+      const T *q = coef + i * (deg + 1);
+      T r = q[0];
+      for (int k = 1; k <= deg; ++k) r = r * z + q[k];
+      return r;
+    }
+    return es_eval(d + (T)i, beta, c, hw);
+  }
+};
+template <typename T> __device__ __forceinline__ Ker<T> make_ker(const Geom &g) {
+  Ker<T> k;
+  k.beta = (T)g.beta;
+  k.c = (T)g.c;
+  k.hw = (T)(0.5 * g.kw);
+  k.deg = g.hdeg;
+  k.flatw = g.kww == 1;
+  if constexpr (sizeof(T) == 8)
+    k.coef = (const T *)g.hcoef_d;
+  else
+    k.coef = (const T *)g.hcoef_f;
+  return k;
+}
+
 __device__ __forceinline__ int imod(int a, int n) {
   int r = a % n;
   return r < 0 ? r + n : r;
@@ -400,7 +435,7 @@ __global__ void __launch_bounds__(256) spread_tile_kernel(const Rec<T> *__restri
   __syncthreads();
 
   const int kw = g.kw, kww = g.kww;
-  const T beta = (T)g.beta, cc = (T)g.c, hw = (T)(0.5 * kw);
+  const Ker<T> K = make_ker<T>(g);
   // source rows: first plane in [p-kww+1, p], first cell in [x0-kw+1, x0+wx-1] x [y0-kw+1, y0+wy-1]
   walk_tiles(imod(p - kww + 1, g.P), kww, g.P, g.Pbin, g.nlayer, [&](int sb) {
     walk_tiles(imod(y0 - kw + 1, g.nf2), wy + kw - 1, g.nf2, g.T2, g.nty, [&](int sty) {
@@ -421,18 +456,18 @@ __global__ void __launch_bounds__(256) spread_tile_kernel(const Rec<T> *__restri
           const bool hitx = (rx < wx) || (rx + kw - 1 >= g.nf1);
           const bool hity = (ry < wy) || (ry + kw - 1 >= g.nf2);
           if (!hitx || !hity) continue;
-          const T vw = es_eval(rc.dw + (T)k, beta, cc, hw);
+          const T vw = K.w(rc.dw, k);
           const T vre = rc.re * vw, vim = rc.im * vw;
           T kx[KWM];
 #pragma unroll
-          for (int i = 0; i < KWM; ++i) kx[i] = (i < kw) ? es_eval(rc.dx + (T)i, beta, cc, hw) : (T)0;
+          for (int i = 0; i < KWM; ++i) kx[i] = (i < kw) ? K(rc.dx, i) : (T)0;
 #pragma unroll
           for (int jj = 0; jj < KWM; ++jj) {
             if (jj >= kw) break;
             int cy = ry + jj;
             if (cy >= g.nf2) cy -= g.nf2;
             if (cy >= wy) continue;
-            const T ky = es_eval(rc.dy + (T)jj, beta, cc, hw);
+            const T ky = K(rc.dy, jj);
             const T are = vre * ky, aim = vim * ky;
             T *rowp = acc + 2 * (cy * g.T1);
 #pragma unroll
@@ -633,7 +668,7 @@ __global__ void __launch_bounds__(256) interp_kernel(const Rec<T> *__restrict__ 
   const long n0 = ra1 - ra0, ntot = n0 + (rb1 - rb0);
   const size_t plane = (size_t)g.nf1 * g.nf2;
   const int kw = g.kw;
-  const T beta = (T)g.beta, cc = (T)g.c, hw = (T)(0.5 * kw);
+  const Ker<T> K = make_ker<T>(g);
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < ntot; t += (long)gridDim.x * blockDim.x) {
     const int r = t < n0 ? ra0 + (int)t : rb0 + (int)(t - n0);
     const Rec<T> rc = recs[r];
@@ -647,15 +682,15 @@ __global__ void __launch_bounds__(256) interp_kernel(const Rec<T> *__restrict__ 
     if (!any) continue;
 #pragma unroll
     for (int i = 0; i < KWM; ++i) {
-      kx[i] = (i < kw) ? es_eval(rc.dx + (T)i, beta, cc, hw) : (T)0;
-      ky[i] = (i < kw) ? es_eval(rc.dy + (T)i, beta, cc, hw) : (T)0;
+      kx[i] = (i < kw) ? K(rc.dx, i) : (T)0;
+      ky[i] = (i < kw) ? K(rc.dy, i) : (T)0;
     }
     double sr = 0.0, si = 0.0;
     for (int k = 0; k < g.kww; ++k) {
       int q = rc.iw + k;
       if (q >= g.P) q -= g.P;
       if (q < p0 || q >= p0 + nb) continue;
-      const T kz = es_eval(rc.dw + (T)k, beta, cc, hw);
+      const T kz = K.w(rc.dw, k);
       const v2_t<T> *pb = grid + (size_t)(q - p0) * plane;
       T pr = 0, pi = 0;
 #pragma unroll
@@ -953,8 +988,8 @@ __device__ __forceinline__ int grid_index(int i, int N, int nf) {
 
 // Spread of one record into the workgroup's grid rows [row0, row0 + rpw) of plane p (LDS).
 template <typename T, int KWM>
-__device__ __forceinline__ void spread_rows(const Rec<T> &rc, T *ldsf, const Geom &g, int p, int row0, T beta,
-                                            T cc, T hw) {
+__device__ __forceinline__ void spread_rows(const Rec<T> &rc, T *ldsf, const Geom &g, int p, int row0,
+                                            const Ker<T> &K) {
   const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2, rpw = g.rpw;
   int k = p - rc.iw;
   if (k < 0) k += g.P;
@@ -962,16 +997,16 @@ __device__ __forceinline__ void spread_rows(const Rec<T> &rc, T *ldsf, const Geo
   int ry = rc.iy - row0;
   if (ry < 0) ry += nf2;
   if (!(ry < rpw || ry + kw - 1 >= nf2)) return;
-  const T vw = es_eval(rc.dw + (T)k, beta, cc, hw);
+  const T vw = K.w(rc.dw, k);
   const T vre = rc.re * vw, vim = rc.im * vw;
   T kx[KWM];
 #pragma unroll
-  for (int i = 0; i < KWM; ++i) kx[i] = (i < kw) ? es_eval(rc.dx + (T)i, beta, cc, hw) : (T)0;
+  for (int i = 0; i < KWM; ++i) kx[i] = (i < kw) ? K(rc.dx, i) : (T)0;
   for (int jj = 0; jj < kw; ++jj) {
     int cy = ry + jj;
     if (cy >= nf2) cy -= nf2;
     if (cy >= rpw) continue;
-    const T ky = es_eval(rc.dy + (T)jj, beta, cc, hw);
+    const T ky = K(rc.dy, jj);
     const T are = vre * ky, aim = vim * ky;
     const int base = cy * nf1;
 #pragma unroll
@@ -991,7 +1026,7 @@ __device__ __forceinline__ void spread_rows(const Rec<T> &rc, T *ldsf, const Geo
 // partial sum over those rows, added to the record's FP64 accumulator.
 template <typename T, int KWM>
 __device__ __forceinline__ void interp_rows(const Rec<T> &rc, int idx, const cpx<T> *lds, const Geom &g, int p,
-                                            int row0, double2 *__restrict__ vacc, T beta, T cc, T hw) {
+                                            int row0, double2 *__restrict__ vacc, const Ker<T> &K) {
   const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2, rpw = g.rpw;
   int k = p - rc.iw;
   if (k < 0) k += g.P;
@@ -1001,13 +1036,13 @@ __device__ __forceinline__ void interp_rows(const Rec<T> &rc, int idx, const cpx
   if (!(ry < rpw || ry + kw - 1 >= nf2)) return;
   T kx[KWM];
 #pragma unroll
-  for (int i = 0; i < KWM; ++i) kx[i] = (i < kw) ? es_eval(rc.dx + (T)i, beta, cc, hw) : (T)0;
+  for (int i = 0; i < KWM; ++i) kx[i] = (i < kw) ? K(rc.dx, i) : (T)0;
   T sr = 0, si = 0;
   for (int jj = 0; jj < kw; ++jj) {
     int cy = ry + jj;
     if (cy >= nf2) cy -= nf2;
     if (cy >= rpw) continue;
-    const T ky = es_eval(rc.dy + (T)jj, beta, cc, hw);
+    const T ky = K(rc.dy, jj);
     const int base = cy * nf1;
     T rr = 0, ri = 0;
 #pragma unroll
@@ -1023,7 +1058,7 @@ __device__ __forceinline__ void interp_rows(const Rec<T> &rc, int idx, const cpx
     sr += rr * ky;
     si += ri * ky;
   }
-  const T kz = es_eval(rc.dw + (T)k, beta, cc, hw);
+  const T kz = K.w(rc.dw, k);
   atomicAdd(&vacc[idx].x, (double)(sr * kz));
   atomicAdd(&vacc[idx].y, (double)(si * kz));
 }
@@ -1088,7 +1123,7 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) spread_row_fft_kernel(co
   if (threadIdx.x < 64) walk_table_build(wt, off, g, p, row0, rpw);
   for (int i = threadIdx.x; i < ntot; i += nt) lds[i] = fftc::mk((T)0, (T)0);
   __syncthreads();
-  const T beta = (T)g.beta, cc = (T)g.c, hw = (T)(0.5 * g.kw);
+  const Ker<T> K = make_ker<T>(g);
   const int total = wt.total;
   const int N1 = g.N1, nout = rpw * N1, h1 = N1 / 2;
   const int lgt = 31 - __clz(g.nf2) + kTileLog2;  // log2(nf2 * kTile)
@@ -1106,8 +1141,8 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) spread_row_fft_kernel(co
     const Rec<T> ra = recs[walk_record(wt, t0)];
     Rec<T> rb;
     if (t1 < total) rb = recs[walk_record(wt, t1)];
-    spread_rows<T, KWM>(ra, ldsf, g, p, row0, beta, cc, hw);
-    if (t1 < total) spread_rows<T, KWM>(rb, ldsf, g, p, row0, beta, cc, hw);
+    spread_rows<T, KWM>(ra, ldsf, g, p, row0, K);
+    if (t1 < total) spread_rows<T, KWM>(rb, ldsf, g, p, row0, K);
   }
   __syncthreads();
   lds_fft_rows<T, E, 16>(lds, 31 - __clz(nf1), rpw, +1, tv);
@@ -1180,11 +1215,11 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) row_ifft_interp_kernel(c
     r0 = recs[idx0];
   }
   lds_fft_rows<T, E, 16>(lds, lg, rpw, sign, tv);
-  const T beta = (T)g.beta, cc = (T)g.c, hw = (T)(0.5 * g.kw);
-  if (threadIdx.x < total) interp_rows<T, KWM>(r0, idx0, lds, g, p, row0, vacc, beta, cc, hw);
+  const Ker<T> K = make_ker<T>(g);
+  if (threadIdx.x < total) interp_rows<T, KWM>(r0, idx0, lds, g, p, row0, vacc, K);
   for (int t = threadIdx.x + nt; t < total; t += nt) {
     const int idx = walk_record(wt, t);
-    interp_rows<T, KWM>(recs[idx], idx, lds, g, p, row0, vacc, beta, cc, hw);
+    interp_rows<T, KWM>(recs[idx], idx, lds, g, p, row0, vacc, K);
   }
 }
 
@@ -1194,7 +1229,7 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) row_ifft_interp_kernel(c
 // decimation in frequency; w = e^{2 pi i / nf1}, m = nf1 / S).
 template <typename T, int KWM, typename Tw>
 __device__ __forceinline__ void spread_row_part(const Rec<T> &rc, T *ldsf, const Tw &tv, const Geom &g, int p,
-                                                int row0, int h, int mm, int sign, T beta, T cc, T hw) {
+                                                int row0, int h, int mm, int sign, const Ker<T> &K) {
   const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2;
   int k = p - rc.iw;
   if (k < 0) k += g.P;
@@ -1202,14 +1237,14 @@ __device__ __forceinline__ void spread_row_part(const Rec<T> &rc, T *ldsf, const
   int jj = row0 - rc.iy;
   if (jj < 0) jj += nf2;
   if (jj >= kw) return;
-  const T a = es_eval(rc.dw + (T)k, beta, cc, hw) * es_eval(rc.dy + (T)jj, beta, cc, hw);
+  const T a = K.w(rc.dw, k) * K(rc.dy, jj);
   const T vre = rc.re * a, vim = rc.im * a;
 #pragma unroll
   for (int i = 0; i < KWM; ++i) {
     if (i < kw) {
       int cx = rc.ix + i;
       if (cx >= nf1) cx -= nf1;
-      const T kx = es_eval(rc.dx + (T)i, beta, cc, hw);
+      const T kx = K(rc.dx, i);
       T re = vre * kx, im = vim * kx;
       if (h) {
         const cpx<T> w = tv[(h * cx) & (nf1 - 1)];
@@ -1260,14 +1295,14 @@ __global__ void __launch_bounds__(MaxThreads<T>::value / S) spread_row_fft_part_
     }
     return;
   }
-  const T beta = (T)g.beta, cc = (T)g.c, hw = (T)(0.5 * g.kw);
+  const Ker<T> K = make_ker<T>(g);
   for (int t0 = threadIdx.x; t0 < total; t0 += 2 * nt) {
     const int t1 = t0 + nt;
     const Rec<T> ra = recs[walk_record(wt, t0)];
     Rec<T> rb;
     if (t1 < total) rb = recs[walk_record(wt, t1)];
-    spread_row_part<T, KWM>(ra, ldsf, tv, g, p, row0, h, m - 1, sign, beta, cc, hw);
-    if (t1 < total) spread_row_part<T, KWM>(rb, ldsf, tv, g, p, row0, h, m - 1, sign, beta, cc, hw);
+    spread_row_part<T, KWM>(ra, ldsf, tv, g, p, row0, h, m - 1, sign, K);
+    if (t1 < total) spread_row_part<T, KWM>(rb, ldsf, tv, g, p, row0, h, m - 1, sign, K);
   }
   __syncthreads();
   lds_fft_part<T, E, 16, S>(lds, 31 - __clz(m), sign, tv);
@@ -1301,7 +1336,7 @@ __global__ void __launch_bounds__(MaxThreads<T>::value / S) spread_row_fft_part_
 template <typename T, int KWM, typename Tw>
 __device__ __forceinline__ void interp_row_part(const Rec<T> &rc, int idx, const cpx<T> *lds, const Tw &tv,
                                                 const Geom &g, int p, int row0, int h, int mm, int sign,
-                                                double2 *__restrict__ vacc, T beta, T cc, T hw) {
+                                                double2 *__restrict__ vacc, const Ker<T> &K) {
   const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2;
   int k = p - rc.iw;
   if (k < 0) k += g.P;
@@ -1315,7 +1350,7 @@ __device__ __forceinline__ void interp_row_part(const Rec<T> &rc, int idx, const
     if (i < kw) {
       int cx = rc.ix + i;
       if (cx >= nf1) cx -= nf1;
-      const T kx = es_eval(rc.dx + (T)i, beta, cc, hw);
+      const T kx = K(rc.dx, i);
       cpx<T> f = lds[fftc::pad(cx & mm)];
       if (h) {
         const cpx<T> w = tv[(h * cx) & (nf1 - 1)];
@@ -1325,7 +1360,7 @@ __device__ __forceinline__ void interp_row_part(const Rec<T> &rc, int idx, const
       ri += f.y * kx;
     }
   }
-  const T kyz = es_eval(rc.dy + (T)jj, beta, cc, hw) * es_eval(rc.dw + (T)k, beta, cc, hw);
+  const T kyz = K(rc.dy, jj) * K.w(rc.dw, k);
   atomicAdd(&vacc[idx].x, (double)(rr * kyz));
   atomicAdd(&vacc[idx].y, (double)(ri * kyz));
 }
@@ -1368,12 +1403,12 @@ __global__ void __launch_bounds__(MaxThreads<T>::value / S) row_ifft_interp_part
     r0 = recs[idx0];
   }
   lds_fft_part<T, E, 16, S>(lds, lgm, sign, tv);
-  const T beta = (T)g.beta, cc = (T)g.c, hw = (T)(0.5 * g.kw);
+  const Ker<T> K = make_ker<T>(g);
   if (threadIdx.x < total)
-    interp_row_part<T, KWM>(r0, idx0, lds, tv, g, p, row0, h, m - 1, sign, vacc, beta, cc, hw);
+    interp_row_part<T, KWM>(r0, idx0, lds, tv, g, p, row0, h, m - 1, sign, vacc, K);
   for (int t = threadIdx.x + nt; t < total; t += nt) {
     const int idx = walk_record(wt, t);
-    interp_row_part<T, KWM>(recs[idx], idx, lds, tv, g, p, row0, h, m - 1, sign, vacc, beta, cc, hw);
+    interp_row_part<T, KWM>(recs[idx], idx, lds, tv, g, p, row0, h, m - 1, sign, vacc, K);
   }
 }
 
@@ -1569,7 +1604,7 @@ namespace {
 // Grow-only device buffers, one set per device (calls on one device are serialised).
 enum Slot {
   S_PART, S_CNT, S_HIST, S_OFF, S_BSUM, S_BINRANK, S_RECS, S_PERM, S_GRID, S_ACC, S_VACC, S_TAB, S_FFTW,
-  S_H, S_TW,
+  S_H, S_TW, S_HC,
   S_NSLOT
 };
 struct Workspace {
@@ -1764,6 +1799,20 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
     return kErrSize;
   }
   Tuning tn = tune ? *tune : Tuning();
+  // kernel evaluation: exp(sqrt()), or the piecewise polynomial (coefficients
+  // uploaded with the other tables when the buffers are reserved)
+  g.hdeg = -1;
+  g.hcoef_d = nullptr;
+  g.hcoef_f = nullptr;
+  if (tn.kereval == 1) {
+    std::vector<double> coef;
+    g.hdeg = horner_fit(ko, horner_tol(eps, dbl), coef);
+    if (g.hdeg < 0) {
+      set_error("no polynomial of degree <= %d fits the kernel (kw %d) to %g", kMaxHornerDeg, ko.kw,
+                horner_tol(eps, dbl));
+      return kErrArg;
+    }
+  }
   g.Pbin = std::max(1, std::min(tn.Pbin > 0 ? tn.Pbin : 4, g.P));
   g.nlayer = (g.P + g.Pbin - 1) / g.Pbin;
   // pruned LDS FFT: power-of-two grids whose rows and columns fit one
@@ -1875,6 +1924,8 @@ struct Bufs {
   void *recs = nullptr, *grid = nullptr, *H = nullptr;
   double2 *acc = nullptr, *vacc = nullptr;
   const void *tw1 = nullptr, *tw2 = nullptr;
+  const double *hc_d = nullptr;  // piecewise-polynomial kernel coefficients, FP64 | FP32
+  const float *hc_f = nullptr;
 };
 
 // Phase 1 (one wait): w range and unflagged count of every problem, then their geometries.
@@ -2057,6 +2108,25 @@ int reserve(const std::vector<ChanPlan> &cps, const Problem *pbs, bool to_image,
   if (!to_image) {
     CURIG_TRY(ws_get(dev, S_VACC, sizeof(double2) * nvalid, &vp));
     b.vacc = (double2 *)vp;
+  }
+  if (g0.hdeg >= 0) {
+    // same fit as the plan's (same options and tolerance), FP64 then FP32 copies
+    std::vector<double> coef;
+    const int deg = horner_fit(cps[0].ko, horner_tol(pbs[0].eps, pbs[0].dbl), coef);
+    if (deg != g0.hdeg) {
+      set_error("kernel polynomial degree changed between plan and upload");
+      return kErrArg;
+    }
+    const size_t nc = coef.size();
+    std::vector<unsigned char> host(nc * (sizeof(double) + sizeof(float)));
+    memcpy(host.data(), coef.data(), nc * sizeof(double));
+    float *cf = (float *)(host.data() + nc * sizeof(double));
+    for (size_t i = 0; i < nc; ++i) cf[i] = (float)coef[i];
+    CURIG_TRY(ws_get(dev, S_HC, host.size(), &vp));
+    CURIG_HIP(hipMemcpyAsync(vp, host.data(), host.size(), hipMemcpyHostToDevice, st));
+    CURIG_HIP(hipStreamSynchronize(st));
+    b.hc_d = (const double *)vp;
+    b.hc_f = (const float *)((const char *)vp + nc * sizeof(double));
   }
   if (g0.fft_lds) {
     CURIG_TRY(ws_get(dev, S_H, sizeof(cpx<T>) * h, &vp));
@@ -2466,6 +2536,10 @@ int run_problems(bool to_image, const Problem *pbs, int n, hipStream_t st, const
   CURIG_TRY(plan_all(pbs, n, st, tune, dev, cps));
   Bufs b;
   CURIG_TRY((reserve<T>(cps, pbs, to_image, dev, st, b)));
+  for (ChanPlan &cp : cps) {
+    cp.g.hcoef_d = b.hc_d;
+    cp.g.hcoef_f = b.hc_f;
+  }
   // graph mode needs every kernel of the sequence to be a plain launch (LDS FFT path)
   bool use_graph = graph_mode != 0;
   for (const ChanPlan &cp : cps) use_graph = use_graph && cp.g.fft_lds;

@@ -1,5 +1,6 @@
 // Host plan math: kernel width / beta, grid sizes, w-plane count, quadrature and
 // the kernel's Fourier series.  Pure host code (no HIP calls).
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -167,6 +168,70 @@ void fseries_grid(const KernelOpts &o, int nf, double *out) {
     for (int i = 0; i < n; ++i) s += g[i] * std::cos((x[i] - half) / half * M_PI * k);
     out[k] = s;
   }
+}
+
+// Piecewise-polynomial form of the normalised kernel (the reference's
+// gpu_kerevalmeth = 1, declared in include/curafft_opts.h:23 but not
+// implemented there).  On each unit interval i, x in [-kw/2 + i, -kw/2 + i + 1),
+// phi is the polynomial sum_k a_k T_k(z), z = 2 t - 1 with t = x + kw/2 - i,
+// interpolating phi at deg + 1 Chebyshev nodes (coefficients by the discrete
+// cosine sum, so the fit is well conditioned), then rewritten in powers of z
+// for Horner's rule.  deg is the smallest degree from kw - 1 up to kMaxHornerDeg
+// whose largest error on 256 points per interval is below tol (phi peaks at 1).
+// Fit tolerance: a quarter of the requested accuracy, not below what the
+// evaluation precision resolves.  (The kernel's square-root edge at |x| = kw/2
+// limits any polynomial fit of the outer intervals to ~1.3 e^-beta, about a
+// tenth of epsilon, whatever the degree.)
+double horner_tol(double eps, bool dbl) { return std::max(0.25 * eps, dbl ? 1e-15 : 3e-8); }
+
+int horner_fit(const KernelOpts &o, double tol, std::vector<double> &coef) {
+  const int kw = o.kw;
+  auto phi = [&](double x) {
+    const double s = 1.0 - o.c * x * x;
+    return (std::fabs(x) < o.halfwidth && s > 0) ? std::exp(-o.beta * o.c * x * x / (std::sqrt(s) + 1.0)) : 0.0;
+  };
+  for (int deg = std::max(1, kw - 1); deg <= kMaxHornerDeg; ++deg) {
+    const int n = deg + 1;
+    coef.assign((size_t)kw * n, 0.0);
+    double err = 0.0;
+    for (int i = 0; i < kw; ++i) {
+      long double a[kMaxHornerDeg + 1] = {0};
+      for (int k = 0; k < n; ++k) {
+        long double sum = 0;
+        for (int j = 0; j < n; ++j) {
+          const long double th = M_PI * (j + 0.5L) / n;
+          const double z = (double)std::cos(th);
+          sum += phi(-0.5 * kw + i + 0.5 * (z + 1.0)) * std::cos(k * th);
+        }
+        a[k] = sum * (k == 0 ? 1.0L : 2.0L) / n;
+      }
+      // sum a_k T_k(z) in powers of z: T_{k+1} = 2 z T_k - T_{k-1}
+      long double tm[kMaxHornerDeg + 2] = {0}, t0[kMaxHornerDeg + 2] = {0}, p[kMaxHornerDeg + 2] = {0};
+      t0[0] = 1;  // T_0
+      tm[1] = 1;  // T_1 (held in tm, T_0 in t0)
+      p[0] += a[0];
+      if (n > 1) p[1] += a[1];
+      for (int k = 2; k < n; ++k) {
+        long double tn[kMaxHornerDeg + 2] = {0};
+        for (int q = 0; q <= k; ++q) tn[q] = (q > 0 ? 2 * tm[q - 1] : 0) - t0[q];
+        for (int q = 0; q <= k; ++q) p[q] += a[k] * tn[q];
+        for (int q = 0; q <= k; ++q) {
+          t0[q] = tm[q];
+          tm[q] = tn[q];
+        }
+      }
+      for (int q = 0; q < n; ++q) coef[(size_t)i * n + q] = (double)p[deg - q];  // highest power first
+      for (int s2 = 0; s2 <= 256; ++s2) {
+        const double z = -1.0 + 2.0 * s2 / 256.0;
+        double r = coef[(size_t)i * n];
+        for (int q = 1; q < n; ++q) r = r * z + coef[(size_t)i * n + q];
+        err = std::max(err, std::fabs(r - phi(-0.5 * kw + i + 0.5 * (z + 1.0))));
+      }
+    }
+    if (err < tol) return deg;
+  }
+  coef.clear();
+  return -1;
 }
 
 }  // namespace curig

@@ -115,6 +115,17 @@ int curig_nifty_plan_info(int nrow_valid, int nxdirty, int nydirty, double pixsi
 /* Host-only: correction factors phihat(k) (-1)^k, k = 0..nf/2, of the kernel
  * normalised to peak 1 (multiply by e^beta for the reference's unnormalised kernel). */
 int curig_kernel_fseries(double epsilon, double sigma, int nf, double *out);
+/* Kernel evaluation of later calls: 0 = exp(beta (sqrt(1 - c x^2) - 1)) per tap
+ * (default), 1 = Horner's rule on a piecewise polynomial fitted at plan time
+ * (the reference's gpu_kerevalmeth = 1, include/curafft_opts.h:23, declared
+ * there without an implementation).  The plan APIs take it from
+ * opts.gpu_kerevalmeth instead. */
+int curig_set_kernel_eval(int method);
+/* Host-only: that polynomial for (epsilon, sigma) at a precision: *deg and
+ * kw (deg + 1) coefficients, interval i = 0..kw-1 covering x in
+ * [-kw/2 + i, -kw/2 + i + 1), powers of z = 2 (x + kw/2 - i) - 1 highest
+ * first; max_coef is the room in coef. */
+int curig_kernel_horner(double epsilon, double sigma, int precision, int *deg, double *coef, int max_coef);
 /* Host-only: n-point Gauss-Legendre rule on [-1, 1]. */
 int curig_gauss_legendre(int n, double *x, double *w);
 

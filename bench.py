@@ -612,6 +612,8 @@ def main():
     ap.add_argument("--no-err", action="store_true")
     ap.add_argument("--no-api", action="store_true", help="skip the host-pointer (PCIe-inclusive) timing")
     ap.add_argument("--no-eps4", action="store_true", help="skip the epsilon 1e-4 run of the same workload")
+    ap.add_argument("--kereval", type=int, default=0,
+                    help="kernel evaluation: 0 exp(sqrt()), 1 piecewise polynomial (Horner)")
     ap.add_argument("--form", default="precision", choices=["precision", "batched", "diff", "nifty", "nufft"])
     ap.add_argument("--lr", type=float, default=0.02, help="Adam step of the diff form")
     ap.add_argument("--traffic", type=float, default=None,
@@ -624,6 +626,11 @@ def main():
               f"path: each GPU would take a disjoint channel set or w-plane range)", file=sys.stderr)
         sys.exit(2)
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    if args.kereval:
+        from curig_amd import _lib
+
+        _lib.check(_lib.load().curig_set_kernel_eval(args.kereval))
+        log(f"[bench] kernel evaluation: piecewise polynomial (Horner)")
     if args.form != "precision":
         result = {"batched": form_batched, "diff": form_diff, "nifty": form_nifty,
                   "nufft": form_nufft}[args.form](args, log)
@@ -696,7 +703,7 @@ def main():
                                f"{cfg['nx']}x{cfg['ny']} image (channels summed), fov {cfg['fov']} deg, "
                                f"eps {cfg['eps']}, {args.precision}",
                    "planes_per_channel": [s["nf3"] for s in stats_by_ch], "grid": stats_by_ch[0]["nf1"],
-                   "kw": stats_by_ch[0]["kw"]},
+                   "kw": stats_by_ch[0]["kw"], "kernel_eval": "horner" if args.kereval else "exp_sqrt"},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_for(kname, args), "launches_per_call": launches,

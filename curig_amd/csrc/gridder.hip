@@ -76,23 +76,41 @@ __device__ __forceinline__ double es_eval(double d, double beta, double c, doubl
 
 // Kernel value at tap i of a record whose first tap lies at offset d (d + i in
 // [-kw/2, kw/2)): exp(sqrt()) evaluation, or Horner's rule on the piecewise
-// polynomial of interval i at z = 2 (d + kw/2) - 1 (hdeg >= 0).
+// polynomial of interval i at z = 2 (d + kw/2) - 1 (hdeg >= 0; coefficients
+// [hdeg + 1][kw], highest power first, so one power of all taps is contiguous).
 template <typename T> struct Ker {
   T beta, c, hw;
   const T *coef;
-  int deg;
+  int deg, kw;
   bool flatw;  // 2-D form: the single plane's weight is 1
   // weight of plane tap k (first tap at offset d)
   __device__ __forceinline__ T w(T d, int k) const { return flatw ? (T)1 : (*this)(d, k); }
   __device__ __forceinline__ T operator()(T d, int i) const {
     if (deg >= 0) {
       const T z = (T)2 * (d + hw) - (T)1;
-      const T *q = coef + i * (deg + 1);
-      T r = q[0];
-      for (int k = 1; k <= deg; ++k) r = r * z + q[k];
+      T r = coef[i];
+      for (int k = 1; k <= deg; ++k) r = r * z + coef[k * kw + i];
       return r;
     }
     return es_eval(d + (T)i, beta, c, hw);
+  }
+  // all kw taps (zero beyond kw): the polynomial's powers outermost, so each
+  // step is KWM independent multiply-adds
+  template <int KWM> __device__ __forceinline__ void taps(T d, T *o) const {
+    if (deg >= 0) {
+      const T z = (T)2 * (d + hw) - (T)1;
+#pragma unroll
+      for (int i = 0; i < KWM; ++i) o[i] = (i < kw) ? coef[i] : (T)0;
+      for (int k = 1; k <= deg; ++k) {
+        const T *q = coef + k * kw;
+#pragma unroll
+        for (int i = 0; i < KWM; ++i)
+          if (i < kw) o[i] = o[i] * z + q[i];
+      }
+      return;
+    }
+#pragma unroll
+    for (int i = 0; i < KWM; ++i) o[i] = (i < kw) ? es_eval(d + (T)i, beta, c, hw) : (T)0;
   }
 };
 template <typename T> __device__ __forceinline__ Ker<T> make_ker(const Geom &g) {
@@ -101,6 +119,7 @@ template <typename T> __device__ __forceinline__ Ker<T> make_ker(const Geom &g) 
   k.c = (T)g.c;
   k.hw = (T)(0.5 * g.kw);
   k.deg = g.hdeg;
+  k.kw = g.kw;
   k.flatw = g.kww == 1;
   if constexpr (sizeof(T) == 8)
     k.coef = (const T *)g.hcoef_d;
@@ -459,8 +478,7 @@ __global__ void __launch_bounds__(256) spread_tile_kernel(const Rec<T> *__restri
           const T vw = K.w(rc.dw, k);
           const T vre = rc.re * vw, vim = rc.im * vw;
           T kx[KWM];
-#pragma unroll
-          for (int i = 0; i < KWM; ++i) kx[i] = (i < kw) ? K(rc.dx, i) : (T)0;
+          K.template taps<KWM>(rc.dx, kx);
 #pragma unroll
           for (int jj = 0; jj < KWM; ++jj) {
             if (jj >= kw) break;
@@ -680,11 +698,8 @@ __global__ void __launch_bounds__(256) interp_kernel(const Rec<T> *__restrict__ 
       if (q >= p0 && q < p0 + nb) any = true;
     }
     if (!any) continue;
-#pragma unroll
-    for (int i = 0; i < KWM; ++i) {
-      kx[i] = (i < kw) ? K(rc.dx, i) : (T)0;
-      ky[i] = (i < kw) ? K(rc.dy, i) : (T)0;
-    }
+    K.template taps<KWM>(rc.dx, kx);
+    K.template taps<KWM>(rc.dy, ky);
     double sr = 0.0, si = 0.0;
     for (int k = 0; k < g.kww; ++k) {
       int q = rc.iw + k;
@@ -1000,8 +1015,7 @@ __device__ __forceinline__ void spread_rows(const Rec<T> &rc, T *ldsf, const Geo
   const T vw = K.w(rc.dw, k);
   const T vre = rc.re * vw, vim = rc.im * vw;
   T kx[KWM];
-#pragma unroll
-  for (int i = 0; i < KWM; ++i) kx[i] = (i < kw) ? K(rc.dx, i) : (T)0;
+  K.template taps<KWM>(rc.dx, kx);
   for (int jj = 0; jj < kw; ++jj) {
     int cy = ry + jj;
     if (cy >= nf2) cy -= nf2;
@@ -1035,8 +1049,7 @@ __device__ __forceinline__ void interp_rows(const Rec<T> &rc, int idx, const cpx
   if (ry < 0) ry += nf2;
   if (!(ry < rpw || ry + kw - 1 >= nf2)) return;
   T kx[KWM];
-#pragma unroll
-  for (int i = 0; i < KWM; ++i) kx[i] = (i < kw) ? K(rc.dx, i) : (T)0;
+  K.template taps<KWM>(rc.dx, kx);
   T sr = 0, si = 0;
   for (int jj = 0; jj < kw; ++jj) {
     int cy = ry + jj;
@@ -1239,12 +1252,14 @@ __device__ __forceinline__ void spread_row_part(const Rec<T> &rc, T *ldsf, const
   if (jj >= kw) return;
   const T a = K.w(rc.dw, k) * K(rc.dy, jj);
   const T vre = rc.re * a, vim = rc.im * a;
+  T kxs[KWM];
+  K.template taps<KWM>(rc.dx, kxs);
 #pragma unroll
   for (int i = 0; i < KWM; ++i) {
     if (i < kw) {
       int cx = rc.ix + i;
       if (cx >= nf1) cx -= nf1;
-      const T kx = K(rc.dx, i);
+      const T kx = kxs[i];
       T re = vre * kx, im = vim * kx;
       if (h) {
         const cpx<T> w = tv[(h * cx) & (nf1 - 1)];
@@ -1345,12 +1360,14 @@ __device__ __forceinline__ void interp_row_part(const Rec<T> &rc, int idx, const
   if (jj < 0) jj += nf2;
   if (jj >= kw) return;
   T rr = 0, ri = 0;
+  T kxs[KWM];
+  K.template taps<KWM>(rc.dx, kxs);
 #pragma unroll
   for (int i = 0; i < KWM; ++i) {
     if (i < kw) {
       int cx = rc.ix + i;
       if (cx >= nf1) cx -= nf1;
-      const T kx = K(rc.dx, i);
+      const T kx = kxs[i];
       cpx<T> f = lds[fftc::pad(cx & mm)];
       if (h) {
         const cpx<T> w = tv[(h * cx) & (nf1 - 1)];
@@ -2117,11 +2134,17 @@ int reserve(const std::vector<ChanPlan> &cps, const Problem *pbs, bool to_image,
       set_error("kernel polynomial degree changed between plan and upload");
       return kErrArg;
     }
+    // device layout [deg + 1][kw]: one power of every tap contiguous
     const size_t nc = coef.size();
+    const int kw = cps[0].ko.kw, np1 = deg + 1;
     std::vector<unsigned char> host(nc * (sizeof(double) + sizeof(float)));
-    memcpy(host.data(), coef.data(), nc * sizeof(double));
+    double *cd = (double *)host.data();
     float *cf = (float *)(host.data() + nc * sizeof(double));
-    for (size_t i = 0; i < nc; ++i) cf[i] = (float)coef[i];
+    for (int i = 0; i < kw; ++i)
+      for (int k = 0; k < np1; ++k) {
+        cd[(size_t)k * kw + i] = coef[(size_t)i * np1 + k];
+        cf[(size_t)k * kw + i] = (float)coef[(size_t)i * np1 + k];
+      }
     CURIG_TRY(ws_get(dev, S_HC, host.size(), &vp));
     CURIG_HIP(hipMemcpyAsync(vp, host.data(), host.size(), hipMemcpyHostToDevice, st));
     CURIG_HIP(hipStreamSynchronize(st));

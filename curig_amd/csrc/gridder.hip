@@ -2055,8 +2055,7 @@ int allow_kernels(const Geom &g) {
   CURIG_TRY(allow_lds(row_ifft_interp_kernel<T, 16, 8>, g.lds_row_bytes));
   CURIG_TRY(allow_lds(row_ifft_interp_kernel<T, 16, 16>, g.lds_row_bytes));
   CURIG_TRY(allow_lds(row_ifft_interp_kernel<T, 16, kMaxKernelWidth>, g.lds_row_bytes));
-  if (g.rsplit == 2) CURIG_TRY((allow_parts<T, 2>(g)));
-  if (g.rsplit == 4) CURIG_TRY((allow_parts<T, 4>(g)));
+  if (g.rsplit == kRowSplit) CURIG_TRY((allow_parts<T, kRowSplit>(g)));
   CURIG_TRY(allow_lds(col_fwd_kernel<T, 16>, g.lds_col_bytes));
   CURIG_TRY(allow_lds(col_inv_kernel<T, 16>, g.lds_col_bytes));
   return kOk;
@@ -2282,12 +2281,8 @@ void launch_interp_parts(const Rec<T> *recs, const int *off, const cpx<T> *H, do
 template <typename T>
 void lds_spread_rows(const Rec<T> *recs, const int *off, const Geom &g, int p0, int nb, cpx<T> *H,
                      const cpx<T> *tw1, hipStream_t st) {
-  if (g.rsplit == 2) {
-    launch_spread_parts<T, 2>(recs, off, H, tw1, g, p0, nb, st);
-    return;
-  }
-  if (g.rsplit == 4) {
-    launch_spread_parts<T, 4>(recs, off, H, tw1, g, p0, nb, st);
+  if (g.rsplit == kRowSplit) {
+    launch_spread_parts<T, kRowSplit>(recs, off, H, tw1, g, p0, nb, st);
     return;
   }
   if (g.kw <= 8)
@@ -2318,12 +2313,8 @@ void lds_cols_backward(const Geom &g, int p0, int nb, const cpx<T> *gT, cpx<T> *
 template <typename T>
 void lds_row_interp(const Rec<T> *recs, const int *off, const cpx<T> *H, double2 *vacc, const cpx<T> *tw1,
                     const Geom &g, int p0, int nb, hipStream_t st) {
-  if (g.rsplit == 2) {
-    launch_interp_parts<T, 2>(recs, off, H, vacc, tw1, g, p0, nb, st);
-    return;
-  }
-  if (g.rsplit == 4) {
-    launch_interp_parts<T, 4>(recs, off, H, vacc, tw1, g, p0, nb, st);
+  if (g.rsplit == kRowSplit) {
+    launch_interp_parts<T, kRowSplit>(recs, off, H, vacc, tw1, g, p0, nb, st);
     return;
   }
   const dim3 grd(g.nf2 / g.rpw, nb);

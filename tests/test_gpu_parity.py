@@ -407,7 +407,7 @@ def test_lds_fft_path_equals_rocfft_path(precision):
 @pytest.mark.parametrize("precision,n", [("fp64", 4096), ("fp32", 8192), ("fp32", 8191)])
 def test_split_rows_equal_rocfft_path(precision, n):
     """Grids whose rows exceed half a CU's LDS (16384-point FP32, 8192-point
-    FP64) run each row as two half-length workgroups; the result equals the
+    FP64) run each row as several shorter workgroups (four by default); the result equals the
     rocFFT path and, on a few visibilities, the FP64 direct sum."""
     lib = _lib()
     say(f"[split rows] {precision} {n}x{n}")

@@ -21,6 +21,7 @@ EXPORTS = [
     "ms2dirty_1", "ms2dirty_2", "dirty2ms_1", "dirty2ms_2",
     "ms2dirty_f32", "dirty2ms_f32", "curig_ms2dirty_dev", "curig_dirty2ms_dev",
     "curig_ms2dirty_multi_dev", "curig_dirty2ms_multi_dev", "curig_explicit_ms2dirty_dev",
+    "curig_explicit_dirty2ms_dev",
     "curig_nifty_ms2dirty", "curig_nifty_dirty2ms", "curig_nufft_dev", "curig_nufft3_dev",
     "curig_last_error", "curig_plan_info", "curig_nifty_plan_info", "curig_kernel_fseries", "curig_gauss_legendre",
     "curig_set_tuning", "curig_set_kernel_eval", "curig_kernel_horner", "curig_set_fft_path", "curig_set_graph_mode", "curig_enable_stats", "curig_get_stats", "curig_release",
@@ -81,6 +82,9 @@ def load():
     lib.curig_explicit_ms2dirty_dev.argtypes = [c_int, c_int, c_int, c_double, c_double, c_void_p, c_void_p,
                                                 c_void_p, c_int, c_long, c_void_p, c_void_p, c_int, c_void_p]
     lib.curig_explicit_ms2dirty_dev.restype = c_int
+    lib.curig_explicit_dirty2ms_dev.argtypes = [c_int, c_int, c_int, c_double, c_double, c_void_p, c_void_p,
+                                                c_void_p, c_int, c_long, c_void_p, c_void_p, c_int, c_void_p]
+    lib.curig_explicit_dirty2ms_dev.restype = c_int
     nifty = [c_int, c_int, _f64, _f64, c_void_p, c_void_p, c_void_p, c_int, c_int, c_double, c_double, c_int, c_int,
              c_double, c_int, c_void_p, c_int]
     for name in ("curig_nifty_ms2dirty", "curig_nifty_dirty2ms"):

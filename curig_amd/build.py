@@ -14,7 +14,9 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libcurafft.so")
 OBJDIR = os.path.join(HERE, "_build")
-SOURCES = ["gridder.hip", "plan_math.cpp", "fft.cpp", "capi.cpp"]
+SOURCES = ["kern_lds_spread_f32.hip", "kern_lds_interp_f32.hip", "kern_lds_cols_f32.hip", "kern_lds_spread_f64.hip",
+           "kern_lds_interp_f64.hip", "kern_lds_cols_f64.hip", "kern_plane.hip", "kern_prep.hip", "gridder.hip",
+           "plan_math.cpp", "fft.cpp", "capi.cpp"]
 ARCH = os.environ.get("CURIG_OFFLOAD_ARCH", "gfx950")
 
 
@@ -35,7 +37,7 @@ def _flags():
     ]
 
 
-def build(verbose=False, jobs=4, defines=(), out=None, objdir=None):
+def build(verbose=False, jobs=None, defines=(), out=None, objdir=None):
     """Compile and link the library.  defines: extra -D macros (e.g. the
     layout constants of curig_internal.h); out / objdir: another library path
     and object folder (default: the package's libcurafft.so)."""
@@ -55,6 +57,7 @@ def build(verbose=False, jobs=4, defines=(), out=None, objdir=None):
             print(r.stderr)
         return obj
 
+    jobs = jobs or min(len(SOURCES), max(1, min(os.cpu_count() or 4, 16)))
     with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(compile_one, SOURCES))
     tmp = out + ".tmp"
@@ -67,5 +70,27 @@ def build(verbose=False, jobs=4, defines=(), out=None, objdir=None):
     return out
 
 
+def build_test_driver(lib=None):
+    """The tests' driver of the C++ plan APIs (tests/plan_driver.cpp): host C++
+    compiled by hipcc into tests/_bin/libcurig_plan_driver.so, linked against
+    the library it drives (found next to it through the rpath)."""
+    lib = lib or LIB
+    src = os.path.join(ROOT, "tests", "plan_driver.cpp")
+    outdir = os.path.join(ROOT, "tests", "_bin")
+    os.makedirs(outdir, exist_ok=True)
+    out = os.path.join(outdir, "libcurig_plan_driver.so")
+    rel = os.path.relpath(os.path.dirname(lib), outdir)
+    cmd = [_hipcc(), "-x", "c++", "-O2", "-std=c++17", "-fPIC", "-shared", "-D__HIP_PLATFORM_AMD__",
+           "-I/opt/rocm/include", "-I", INCLUDE, src, "-o", out + ".tmp",
+           "-L", os.path.dirname(lib), "-l:" + os.path.basename(lib), f"-Wl,-rpath,$ORIGIN/{rel}",
+           "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(out + ".tmp", out)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_test_driver())

@@ -584,6 +584,20 @@ extern "C" int curig_explicit_ms2dirty_dev(int nrow, int nxdirty, int nydirty, d
   return explicit_ms2dirty_device(pb, npix, d_pix, d_out, (hipStream_t)stream);
 }
 
+extern "C" int curig_explicit_dirty2ms_dev(int nrow, int nxdirty, int nydirty, double fov, double freq,
+                                           const double *d_uvw, const void *d_dirty, const void *d_wgt, int precision,
+                                           long nsel, const int64_t *d_rows, void *d_out, int sign, void *stream) {
+  CURIG_TRY(check_precision(precision));
+  if (!d_out || !d_dirty) {
+    set_error("NULL image or output");
+    return kErrArg;
+  }
+  Problem pb = dev_problem(nrow, nxdirty, nydirty, fov, freq, d_uvw, nullptr, d_wgt, d_dirty, 1e-6, 2.0, sign,
+                           precision);
+  pb.vis = d_out;  // unused by the direct sum; satisfies the argument check
+  return explicit_dirty2ms_device(pb, nsel, d_rows, d_out, (hipStream_t)stream);
+}
+
 extern "C" const char *curig_last_error(void) { return last_error(); }
 
 extern "C" int curig_plan_info(int nrow_valid, int nxdirty, int nydirty, double fov, double freq, double epsilon,
@@ -771,10 +785,16 @@ int gridder_setting(int N1, int N2, int method, int kerevalmeth, int w_term_meth
     set_error("NULL plan or visibility struct");
     return kErrArg;
   }
-  if (w_term_method != 1 || method != 0 || kerevalmeth != 0) {
-    set_error("only improved w-stacking (w_term_method 1), method 0, kerevalmeth 0 are provided");
+  if (w_term_method != 1 || method != 0 || (kerevalmeth != 0 && kerevalmeth != 1)) {
+    set_error("only improved w-stacking (w_term_method 1), method 0 and kerevalmeth 0 (exp) or 1 (Horner) "
+              "are provided");
     return kErrArg;
   }
+  if (plan->opts.precision != 0 && plan->opts.precision != 1) {
+    set_error("opts.precision must be 0 (FP64) or 1 (FP32-mixed)");
+    return kErrArg;
+  }
+  const bool dbl = plan->opts.precision == 0;
   KernelOpts ko;
   int ier = setup_conv_opts(ko, tol, sigma);
   if (ier != kOk) return ier;
@@ -783,6 +803,9 @@ int gridder_setting(int N1, int N2, int method, int kerevalmeth, int w_term_meth
     return M < 0 ? kErrNrow : kErrSize;
   }
   plan->tol = tol;
+  plan->opts.upsampfac = sigma;
+  plan->opts.gpu_kerevalmeth = kerevalmeth;  // cugridder.cu:146
+  plan->opts.gpu_gridder_method = method;
   plan->copts.kw = ko.kw;
   plan->copts.direction = direction;
   plan->copts.pirange = 1;
@@ -790,7 +813,6 @@ int gridder_setting(int N1, int N2, int method, int kerevalmeth, int w_term_meth
   plan->copts.ES_beta = ko.beta;
   plan->copts.ES_halfwidth = ko.halfwidth;
   plan->copts.ES_c = ko.c;
-  plan->opts.upsampfac = sigma;
   plan->iflag = direction > 0 ? 1 : -1;
   plan->type = direction > 0 ? 1 : 2;
   plan->dim = 3;
@@ -822,8 +844,14 @@ int gridder_setting(int N1, int N2, int method, int kerevalmeth, int w_term_meth
     CURIG_HIP(hipGetLastError());
   }
   if (pointer_v->weight && M > 0) {
+    // the weights in the precision of the plan's data
     CURIG_HIP(hipMalloc(&plan->d_wgt, sizeof(double) * M));
-    CURIG_HIP(hipMemcpy(plan->d_wgt, pointer_v->weight, sizeof(double) * M, hipMemcpyHostToDevice));
+    if (dbl) {
+      CURIG_HIP(hipMemcpy(plan->d_wgt, pointer_v->weight, sizeof(double) * M, hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> wf(pointer_v->weight, pointer_v->weight + M);
+      CURIG_HIP(hipMemcpy(plan->d_wgt, wf.data(), sizeof(float) * M, hipMemcpyHostToDevice));
+    }
   }
   // plan sizes for the caller (w range from the host copy when present)
   double wmin = 0, wmax = 0;
@@ -837,8 +865,10 @@ int gridder_setting(int N1, int N2, int method, int kerevalmeth, int w_term_meth
     }
   }
   Geom g;
-  ier = plan_geometry(M, N2, N1, fov, pointer_v->frequency[0], tol, sigma, pointer_v->sign, wmin, wmax,
-                      plan->opts.precision == 0, &g_tuning, &g, nullptr);
+  Tuning tn = g_tuning;
+  tn.kereval = kerevalmeth;
+  ier = plan_geometry(M, N2, N1, fov, pointer_v->frequency[0], tol, sigma, pointer_v->sign, wmin, wmax, dbl, &tn, &g,
+                      nullptr);
   if (ier != kOk) return ier;
   plan->nf1 = g.nf1;
   plan->nf2 = g.nf2;
@@ -865,7 +895,7 @@ int gridder_execution(curafft_plan *plan, ragridder_plan *gridder_plan) {
   pb.freq = gridder_plan->kv.frequency[0];
   pb.sigma = plan->copts.upsampfac;
   pb.sign = gridder_plan->kv.sign;
-  pb.dbl = true;
+  pb.dbl = plan->opts.precision == 0;  // as gridder_setting planned
   pb.uvw = plan->d_uvw;
   pb.vis = plan->d_c;
   pb.wgt = plan->d_wgt;

@@ -82,7 +82,7 @@ struct Geom {
   int kw;
   int kww;            // kernel width along w: kw, or 1 in the 2-D form (one plane, weight 1)
   int hdeg;           // >= 0: kernel values by Horner's rule on the piecewise polynomial below
-  const double *hcoef_d;  // [kw][hdeg + 1] coefficients (device), FP64 and FP32 copies
+  const double *hcoef_d;  // [hdeg + 1][kw] coefficients (device; one power of every tap contiguous), FP64 and FP32 copies
   const float *hcoef_f;
   double beta, c;     // normalised ES kernel exp(beta (sqrt(1 - c d^2) - 1))
   double dx, dy;      // pixel sizes (rad): dx pairs with u / image rows, dy with v / columns

@@ -65,6 +65,11 @@ int multi_device(bool to_image, const Problem *pbs, int n, hipStream_t stream, c
 int explicit_ms2dirty_device(const Problem &pb, long npix, const int64_t *pix, void *out,
                              hipStream_t stream);
 
+// FP64 direct sum of the adjoint (dirty2ms) on the GPU at nsel chosen rows:
+// out[q] (complex128) = wgt_j sum_pix dirty[pix] / n e^{-i ph(j, pix)}, j = rows[q]
+// (the conjugate phase of explicit_ms2dirty_device; flags are not consulted).
+int explicit_dirty2ms_device(const Problem &pb, long nsel, const int64_t *rows, void *out, hipStream_t stream);
+
 // Host-only plan computation used by the tests (no device access).
 int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, double eps, double sigma,
                   int sign, double wmin_m, double wmax_m, bool dbl, const Tuning *tune, Geom *g,

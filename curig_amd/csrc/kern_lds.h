@@ -1,0 +1,896 @@
+// Pruned LDS FFT path (power-of-two grids): the row and column kernels of
+// both directions, included by one source per precision and kernel group
+// (kern_lds_{spread,interp,cols}_{f32,f64}.hip), each of which instantiates
+// its launchers.
+#pragma once
+#include "dev_common.h"
+#include "trig.h"
+
+namespace curig {
+
+// --------------------------------------------------------------------------
+// pruned LDS FFT path (power-of-two grids)
+//
+// ms2dirty per batch of planes: (1) one workgroup per grid row of a plane
+// spreads the row's visibilities into LDS, runs the row FFT (length nf1) and
+// writes the N1 image columns to the column-tiled buffer H; (2) one workgroup
+// per image column runs the column FFTs (length nf2) of all the batch's planes
+// and accumulates the N2 image rows with their w phases in FP64.  dirty2ms runs
+// the mirror image: (1) per image column, w-fill + inverse column FFT of every
+// plane -> H; (2) per grid row, inverse row FFT (the row expanded to nf1 cells)
+// and interpolation.  Each plane crosses HBM twice (written once, read once) at
+// half the grid size.
+// --------------------------------------------------------------------------
+
+// Record ranges of one row workgroup (plane p, grid rows [row0, row0 + nrows)):
+// the bins of every layer holding a first plane in [p - kw + 1, p] and of every
+// strip overlapping rows [row0 - kw + 1, row0 + nrows - 1] (both cyclic), with
+// exclusive prefix counts, so that the workgroup walks them as one flat index
+// range and issues all its record loads at once.
+struct WalkTable {
+  int nb, total;
+  int start[kMaxWalk], pre[kMaxWalk];
+};
+
+// Called by the 64 lanes of wave 0; the table is read after a barrier.
+__device__ __forceinline__ void walk_table_build(WalkTable &wt, const int *__restrict__ off, const Geom &g, int p,
+                                                 int row0, int nrows) {
+  const int lane = threadIdx.x;
+  const int al = imod(p - g.kww + 1, g.P), as = imod(row0 - g.kw + 1, g.nf2);
+  const int nl = tiles_count(al, g.kww, g.P, g.Pbin, g.nlayer);
+  const int ns = tiles_count(as, nrows + g.kw - 1, g.nf2, g.T2, g.nty);
+  int nb = nl * ns;
+  if (nb > kMaxWalk) nb = kMaxWalk;  // the plan keeps nb within kMaxWalk
+  const int l0 = al / g.Pbin, s0 = as / g.T2;
+  int carry = 0;
+  for (int base = 0; base < nb; base += 64) {
+    const int k = base + lane;
+    int st = 0, len = 0;
+    if (k < nb) {
+      const int j = k / ns, i = k - j * ns;
+      int layer = l0 + j, strip = s0 + i;
+      if (layer >= g.nlayer) layer -= g.nlayer;
+      if (strip >= g.nty) strip -= g.nty;
+      const int b = layer * g.ntiles + strip;
+      st = off[b];
+      len = off[b + 1] - st;
+    }
+    int x = len;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int y = __shfl_up(x, d, 64);
+      if (lane >= d) x += y;
+    }
+    if (k < nb) {
+      wt.start[k] = st;
+      wt.pre[k] = carry + x - len;
+    }
+    carry += __shfl(x, 63, 64);
+  }
+  if (lane == 0) {
+    wt.nb = nb;
+    wt.total = carry;
+  }
+}
+
+// Record index of flat position t (0 <= t < total): the last range starting at or before t.
+__device__ __forceinline__ int walk_record(const WalkTable &wt, int t) {
+  int lo = 0, hi = wt.nb - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (wt.pre[mid] <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return wt.start[lo] + (t - wt.pre[lo]);
+}
+
+// Copies a two-level twiddle table (hi | lo, fftc::twiddle_entries(n) entries)
+// from global memory to LDS at dst; the caller's next barrier publishes it.
+template <typename T>
+__device__ __forceinline__ fftc::TwoLevel<T, const cpx<T> *> lds_twiddles(cpx<T> *dst,
+                                                                         const cpx<T> *__restrict__ src, int n) {
+  const int lb = fftc::twiddle_split(n), nhi = n >> lb, cnt = nhi + (1 << lb);
+  for (int i = threadIdx.x; i < cnt; i += blockDim.x) dst[i] = src[i];
+  return fftc::TwoLevel<T, const cpx<T> *>{dst, dst + nhi, lb};
+}
+
+template <typename T, int E, int MAXR = 16, typename Tw>
+__device__ __forceinline__ void lds_fft(cpx<T> *buf, const fftc::Layout &L, int sign, const Tw tw, int n_tw) {
+  cpx<T> v[E];
+  const int nt = blockDim.x;
+  for (int Ns = 1; Ns < L.n;) {
+    const int R = fftc::stage_radix(L.n, Ns, MAXR);
+    // opaque copy of the thread id: keeps each stage's address arithmetic inside
+    // the stage (hoisted out of the loop it is held for all stages and spills)
+    int tid = threadIdx.x;
+    __asm__ volatile("" : "+v"(tid));
+    if (R == 16)
+      fftc::stage_load<T, E, 16>(buf, v, L, Ns, sign, tw, n_tw, tid, nt);
+    else if (R == 8)
+      fftc::stage_load<T, E, 8>(buf, v, L, Ns, sign, tw, n_tw, tid, nt);
+    else if (R == 4)
+      fftc::stage_load<T, E, 4>(buf, v, L, Ns, sign, tw, n_tw, tid, nt);
+    else
+      fftc::stage_load<T, E, 2>(buf, v, L, Ns, sign, tw, n_tw, tid, nt);
+    __syncthreads();
+    if (R == 16)
+      fftc::stage_store<T, E, 16>(buf, v, L, Ns, tid, nt);
+    else if (R == 8)
+      fftc::stage_store<T, E, 8>(buf, v, L, Ns, tid, nt);
+    else if (R == 4)
+      fftc::stage_store<T, E, 4>(buf, v, L, Ns, tid, nt);
+    else
+      fftc::stage_store<T, E, 2>(buf, v, L, Ns, tid, nt);
+    __syncthreads();
+    Ns *= R;
+  }
+}
+
+// Stockham stages of `count` contiguous length-N sequences (buffer index
+// pad(s * N + k)) with N a compile-time constant: every stage's radix, stride
+// and twiddle step fold to constants, and the element indexing to shifts.
+// (the twiddle table has TWM * N entries)
+template <typename T, int E, int MAXR, int N, int NS, int TWM = 1, typename Tw>
+__device__ __forceinline__ void fixed_stages(cpx<T> *buf, int count, int sign, const Tw &tw) {
+  if constexpr (NS < N) {
+    constexpr int R = (NS * MAXR <= N) ? MAXR : N / NS;
+    const fftc::Layout L{N, count, N, 1, false};
+    const int nt = blockDim.x;
+    int tid = threadIdx.x;
+    __asm__ volatile("" : "+v"(tid));
+    cpx<T> v[E];
+    fftc::stage_load<T, E, R>(buf, v, L, NS, sign, tw, N * TWM, tid, nt);
+    __syncthreads();
+    fftc::stage_store<T, E, R>(buf, v, L, NS, tid, nt);
+    __syncthreads();
+    fixed_stages<T, E, MAXR, N, NS * R, TWM>(buf, count, sign, tw);
+  }
+}
+
+// FFT of one sequence of length m = 2^lgm (m = nf / S of a row split S ways)
+// with the twiddle table of the full length nf = S m.
+template <typename T, int E, int MAXR, int S, typename Tw>
+__device__ __forceinline__ void lds_fft_part(cpx<T> *buf, int lgm, int sign, const Tw &tw) {
+  switch (lgm) {
+    case 10: fixed_stages<T, E, MAXR, 1024, 1, S>(buf, 1, sign, tw); break;
+    case 11: fixed_stages<T, E, MAXR, 2048, 1, S>(buf, 1, sign, tw); break;
+    case 12: fixed_stages<T, E, MAXR, 4096, 1, S>(buf, 1, sign, tw); break;
+    default: fixed_stages<T, E, MAXR, 8192, 1, S>(buf, 1, sign, tw); break;
+  }
+}
+
+// FFT of `count` contiguous sequences of length n = 2^lg (6 <= lg <= 14) in LDS.
+template <typename T, int E, int MAXR, typename Tw>
+__device__ __forceinline__ void lds_fft_rows(cpx<T> *buf, int lg, int count, int sign, const Tw &tw) {
+  switch (lg) {
+    case 6: fixed_stages<T, E, MAXR, 64, 1>(buf, count, sign, tw); break;
+    case 7: fixed_stages<T, E, MAXR, 128, 1>(buf, count, sign, tw); break;
+    case 8: fixed_stages<T, E, MAXR, 256, 1>(buf, count, sign, tw); break;
+    case 9: fixed_stages<T, E, MAXR, 512, 1>(buf, count, sign, tw); break;
+    case 10: fixed_stages<T, E, MAXR, 1024, 1>(buf, count, sign, tw); break;
+    case 11: fixed_stages<T, E, MAXR, 2048, 1>(buf, count, sign, tw); break;
+    case 12: fixed_stages<T, E, MAXR, 4096, 1>(buf, count, sign, tw); break;
+    case 13: fixed_stages<T, E, MAXR, 8192, 1>(buf, count, sign, tw); break;
+    default: fixed_stages<T, E, MAXR, 16384, 1>(buf, count, sign, tw); break;
+  }
+}
+
+__device__ __forceinline__ int image_index(int k, int N, int nf) {
+  // grid index (FFT order) -> image index (centred), or -1 outside the image
+  const int h = N / 2;
+  return k < N - h ? k + h : (k >= nf - h ? k - nf + h : -1);
+}
+__device__ __forceinline__ int grid_index(int i, int N, int nf) {
+  const int h = N / 2;
+  return i >= h ? i - h : nf + i - h;
+}
+
+// Spread of one record into the workgroup's grid rows [row0, row0 + rpw) of plane p (LDS).
+template <typename T, int KWM>
+__device__ __forceinline__ void spread_rows(const Rec<T> &rc, T *ldsf, const Geom &g, int p, int row0,
+                                            const Ker<T> &K) {
+  const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2, rpw = g.rpw;
+  int k = p - rc.iw;
+  if (k < 0) k += g.P;
+  if (k >= g.kww) return;
+  int ry = rc.iy - row0;
+  if (ry < 0) ry += nf2;
+  if (!(ry < rpw || ry + kw - 1 >= nf2)) return;
+  const T vw = K.w(rc.dw, k);
+  const T vre = rc.re * vw, vim = rc.im * vw;
+  T kx[KWM];
+  K.template taps<KWM>(rc.dx, kx);
+  for (int jj = 0; jj < kw; ++jj) {
+    int cy = ry + jj;
+    if (cy >= nf2) cy -= nf2;
+    if (cy >= rpw) continue;
+    const T ky = K(rc.dy, jj);
+    const T are = vre * ky, aim = vim * ky;
+    const int base = cy * nf1;
+#pragma unroll
+    for (int i = 0; i < KWM; ++i) {
+      if (i < kw) {
+        int cx = rc.ix + i;
+        if (cx >= nf1) cx -= nf1;
+        const int a = 2 * fftc::pad(base + cx);
+        atomicAdd(ldsf + a, are * kx[i]);
+        atomicAdd(ldsf + a + 1, aim * kx[i]);
+      }
+    }
+  }
+}
+
+// Interpolation of one record from the workgroup's rows of plane p (LDS): its
+// partial sum over those rows, added to the record's FP64 accumulator.
+template <typename T, int KWM>
+__device__ __forceinline__ void interp_rows(const Rec<T> &rc, int idx, const cpx<T> *lds, const Geom &g, int p,
+                                            int row0, double2 *__restrict__ vacc, const Ker<T> &K) {
+  const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2, rpw = g.rpw;
+  int k = p - rc.iw;
+  if (k < 0) k += g.P;
+  if (k >= g.kww) return;
+  int ry = rc.iy - row0;
+  if (ry < 0) ry += nf2;
+  if (!(ry < rpw || ry + kw - 1 >= nf2)) return;
+  T kx[KWM];
+  K.template taps<KWM>(rc.dx, kx);
+  T sr = 0, si = 0;
+  for (int jj = 0; jj < kw; ++jj) {
+    int cy = ry + jj;
+    if (cy >= nf2) cy -= nf2;
+    if (cy >= rpw) continue;
+    const T ky = K(rc.dy, jj);
+    const int base = cy * nf1;
+    T rr = 0, ri = 0;
+#pragma unroll
+    for (int i = 0; i < KWM; ++i) {
+      if (i < kw) {
+        int cx = rc.ix + i;
+        if (cx >= nf1) cx -= nf1;
+        const cpx<T> f = lds[fftc::pad(base + cx)];
+        rr += f.x * kx[i];
+        ri += f.y * kx[i];
+      }
+    }
+    sr += rr * ky;
+    si += ri * ky;
+  }
+  const T kz = K.w(rc.dw, k);
+  atomicAdd(&vacc[idx].x, (double)(sr * kz));
+  atomicAdd(&vacc[idx].y, (double)(si * kz));
+}
+
+
+// The intermediate planes live in a column-tiled layout: element (plane k,
+// grid row k2, column slot s) at ((k * nct + s / kTile) * nf2 + k2) * kTile
+// + s % kTile, so a row's kTile-column chunks are contiguous for the row
+// kernels and a column's elements are kTile apart for the column kernels (the
+// kTile columns of a tile run on one XCD, whose L2 serves the shared sectors).
+
+// Column slots of the tiled layout.  With whole rows the slot of image column
+// col is col.  With rows split S ways, the columns of part h (grid index h mod
+// S, col = S i + c_h with c_h = (h + N1/2) mod S) take slots h M + i, M = nf1 /
+// (2 S) >= ceil(N1 / S), so a part writes whole kTile-column chunks; slots whose
+// i is past the part's last column are unused (-1 here).
+__device__ __forceinline__ int slot_column(int s, const Geom &g) {
+  if (g.rsplit <= 1) return s < g.N1 ? s : -1;
+  if (s >= g.nslot) return -1;
+  const int S = g.rsplit, h = s >> g.slot_lg, i = s & ((1 << g.slot_lg) - 1);
+  const int col = S * i + ((h + g.N1 / 2) & (S - 1));
+  return col < g.N1 ? col : -1;
+}
+
+// whether a column workgroup's slots [s0, s0 + cpw) hold any image column
+__device__ __forceinline__ bool slots_used(int s0, const Geom &g) {
+  for (int c = 0; c < g.cpw; ++c)
+    if (slot_column(s0 + c, g) >= 0) return true;
+  return false;
+}
+
+// Workgroup id -> work item such that each XCD (workgroups are dispatched to
+// the 8 XCDs round robin) takes one contiguous range of items.
+__device__ __forceinline__ int xcd_group(int bid, int n) {
+  const int q = n >> 3, r = n & 7, x = bid & 7, k = bid >> 3;
+  return x * q + (x < r ? x : r) + k;
+}
+
+template <typename T> struct MaxThreads { static constexpr int value = sizeof(T) == 8 ? 512 : 1024; };
+
+// ms2dirty: spread into full grid rows held in LDS, then the row FFT (length
+// nf1), keeping the N1 image columns, written to the column-tiled layout H.  One
+// workgroup owns rpw rows of one w plane (rpw * nf1 = E * blockDim.x); rows are
+// binned into full-width strips of T2 rows.
+template <typename T, int E, int KWM>
+__global__ void __launch_bounds__(MaxThreads<T>::value) spread_row_fft_kernel(const Rec<T> *__restrict__ recs,
+                                                                             const int *__restrict__ off,
+                                                                             cpx<T> *__restrict__ H,
+                                                                             const cpx<T> *__restrict__ tw, Geom g,
+                                                                             int p0) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ WalkTable wt;
+  cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
+  T *ldsf = reinterpret_cast<T *>(smem_raw);
+  const int rpw = g.rpw, nf1 = g.nf1, nt = blockDim.x;
+  // rows in XCD-contiguous order: the rows sharing a tile's lines are written
+  // through one L2, which merges their chunks into whole lines
+  const int row0 = xcd_group(blockIdx.x, gridDim.x) * rpw, pl = blockIdx.y, p = p0 + pl;
+  const int ntot = fftc::padded_size(rpw * nf1);
+  const auto tv = lds_twiddles<T>(lds + ntot, tw, nf1);
+  if (threadIdx.x < 64) walk_table_build(wt, off, g, p, row0, rpw);
+  for (int i = threadIdx.x; i < ntot; i += nt) lds[i] = fftc::mk((T)0, (T)0);
+  __syncthreads();
+  const Ker<T> K = make_ker<T>(g);
+  const int total = wt.total;
+  const int N1 = g.N1, nout = rpw * N1, h1 = N1 / 2;
+  const int lgt = 31 - __clz(g.nf2) + kTileLog2;  // log2(nf2 * kTile)
+  cpx<T> *hb = H + ((size_t)pl * g.nct * g.nf2 + row0) * kTile;
+  if (total == 0) {
+    // no visibility reaches these rows of this plane: the transform is zero
+    for (int i = threadIdx.x; i < nout; i += nt) {
+      const int r = rpw == 1 ? 0 : i / N1, col = i - r * N1;
+      hb[((col >> kTileLog2) << lgt) + (r << kTileLog2) + (col & (kTile - 1))] = fftc::mk((T)0, (T)0);
+    }
+    return;
+  }
+  for (int t0 = threadIdx.x; t0 < total; t0 += 2 * nt) {
+    const int t1 = t0 + nt;
+    const Rec<T> ra = recs[walk_record(wt, t0)];
+    Rec<T> rb;
+    if (t1 < total) rb = recs[walk_record(wt, t1)];
+    spread_rows<T, KWM>(ra, ldsf, g, p, row0, K);
+    if (t1 < total) spread_rows<T, KWM>(rb, ldsf, g, p, row0, K);
+  }
+  __syncthreads();
+  lds_fft_rows<T, E, 16>(lds, 31 - __clz(nf1), rpw, +1, tv);
+  // pruned write: all LDS reads first, then the stores (kTile-column chunks);
+  // offsets from the row's base in 32 bits (a plane holds < 2^31 elements)
+  int tid = threadIdx.x;
+  __asm__ volatile("" : "+v"(tid));
+  cpx<T> o[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = tid + e * nt;
+    if (i < nout) {
+      const int r = rpw == 1 ? 0 : i / N1, col = i - r * N1;
+      const int k1 = col >= h1 ? col - h1 : nf1 + col - h1;
+      o[e] = lds[fftc::pad(r * nf1 + k1)];
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = tid + e * nt;
+    if (i < nout) {
+      const int r = rpw == 1 ? 0 : i / N1, col = i - r * N1;
+      hb[((col >> kTileLog2) << lgt) + (r << kTileLog2) + (col & (kTile - 1))] = o[e];
+    }
+  }
+}
+
+// dirty2ms: row pass of the column-transformed planes (H as col_inv_kernel
+// writes it; each row expanded to nf1 cells, zero outside the image columns),
+// inverse row FFT in LDS, then interpolation of every record touching the workgroup's rows of
+// plane p: per record a partial sum, added to its FP64 accumulator (atomics),
+// so the full grid planes are never written.
+template <typename T, int E, int KWM>
+__global__ void __launch_bounds__(MaxThreads<T>::value) row_ifft_interp_kernel(const cpx<T> *__restrict__ H,
+                                                                              const Rec<T> *__restrict__ recs,
+                                                                              const int *__restrict__ off,
+                                                                              double2 *__restrict__ vacc,
+                                                                              const cpx<T> *__restrict__ tw,
+                                                                              Geom g, int p0, int sign) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ WalkTable wt;
+  cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
+  const int rpw = g.rpw, nf1 = g.nf1, N1 = g.N1, nt = blockDim.x;
+  // rows in XCD-contiguous order: the 16 rows sharing a column line run together
+  const int row0 = xcd_group(blockIdx.x, gridDim.x) * rpw, pl = blockIdx.y, p = p0 + pl;
+  const int lg = 31 - __clz(nf1), lg2 = 31 - __clz(g.nf2);
+  cpx<T> v[E];
+  const cpx<T> *hb = H + ((size_t)pl * g.nct * g.nf2 + row0) * kTile;
+  const int lgt = lg2 + kTileLog2;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = threadIdx.x + e * nt;  // < rpw * nf1 = E * nt
+    const int r = i >> lg, k1 = i & (nf1 - 1);
+    const int col = image_index(k1, N1, nf1);
+    v[e] = col >= 0 ? hb[((col >> kTileLog2) << lgt) + (r << kTileLog2) + (col & (kTile - 1))]
+                    : fftc::mk((T)0, (T)0);
+  }
+  const auto tv = lds_twiddles<T>(lds + fftc::padded_size(rpw * nf1), tw, nf1);
+  if (threadIdx.x < 64) walk_table_build(wt, off, g, p, row0, rpw);
+#pragma unroll
+  for (int e = 0; e < E; ++e) lds[fftc::pad(threadIdx.x + e * nt)] = v[e];
+  __syncthreads();
+  const int total = wt.total;
+  if (total == 0) return;  // no visibility reaches these rows of this plane
+  // first record of this thread, loaded while the FFT runs
+  int idx0 = 0;
+  Rec<T> r0;
+  if (threadIdx.x < total) {
+    idx0 = walk_record(wt, threadIdx.x);
+    r0 = recs[idx0];
+  }
+  lds_fft_rows<T, E, 16>(lds, lg, rpw, sign, tv);
+  const Ker<T> K = make_ker<T>(g);
+  if (threadIdx.x < total) interp_rows<T, KWM>(r0, idx0, lds, g, p, row0, vacc, K);
+  for (int t = threadIdx.x + nt; t < total; t += nt) {
+    const int idx = walk_record(wt, t);
+    interp_rows<T, KWM>(recs[idx], idx, lds, g, p, row0, vacc, K);
+  }
+}
+
+// Spread of one record into part h of a row split S ways (ms2dirty): part h
+// holds z_h[n mod m] = sum over the row's cells n of x[n] w^(sign h n), whose
+// m-point FFT gives the row transform at the indices S k + h (radix-S
+// decimation in frequency; w = e^{2 pi i / nf1}, m = nf1 / S).
+template <typename T, int KWM, typename Tw>
+__device__ __forceinline__ void spread_row_part(const Rec<T> &rc, T *ldsf, const Tw &tv, const Geom &g, int p,
+                                                int row0, int h, int mm, int sign, const Ker<T> &K) {
+  const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2;
+  int k = p - rc.iw;
+  if (k < 0) k += g.P;
+  if (k >= g.kww) return;
+  int jj = row0 - rc.iy;
+  if (jj < 0) jj += nf2;
+  if (jj >= kw) return;
+  const T a = K.w(rc.dw, k) * K(rc.dy, jj);
+  const T vre = rc.re * a, vim = rc.im * a;
+  T kxs[KWM];
+  K.template taps<KWM>(rc.dx, kxs);
+#pragma unroll
+  for (int i = 0; i < KWM; ++i) {
+    if (i < kw) {
+      int cx = rc.ix + i;
+      if (cx >= nf1) cx -= nf1;
+      const T kx = kxs[i];
+      T re = vre * kx, im = vim * kx;
+      if (h) {
+        const cpx<T> w = tv[(h * cx) & (nf1 - 1)];
+        const T ws = (T)sign * w.y, t = re * w.x - im * ws;
+        im = re * ws + im * w.x;
+        re = t;
+      }
+      const int q = 2 * fftc::pad(cx & mm);
+      atomicAdd(ldsf + q, re);
+      atomicAdd(ldsf + q + 1, im);
+    }
+  }
+}
+
+// ms2dirty row pass for rows too long for two workgroups per CU: one workgroup
+// per part h of a row split S ways (the row transform's outputs S k + h, see
+// spread_row_part), each spreading the row's visibilities into an m-point LDS
+// buffer, transforming it and writing its share of the N1 image columns (those
+// whose grid index is h mod S).  Several such workgroups fit one CU, so one's
+// memory phases overlap another's FFT.
+template <typename T, int E, int KWM, int S>
+__global__ void __launch_bounds__(MaxThreads<T>::value / S) spread_row_fft_part_kernel(
+    const Rec<T> *__restrict__ recs, const int *__restrict__ off, cpx<T> *__restrict__ H,
+    const cpx<T> *__restrict__ tw, Geom g, int p0) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ WalkTable wt;
+  cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
+  T *ldsf = reinterpret_cast<T *>(smem_raw);
+  const int nf1 = g.nf1, m = nf1 / S, nt = blockDim.x, sign = +1;
+  // the parts of a row are consecutive items: same XCD, adjacent in time
+  const int item = xcd_group(blockIdx.x, gridDim.x);
+  const int row0 = item / S, h = item % S, pl = blockIdx.y, p = p0 + pl;
+  const int ntot = fftc::padded_size(m);
+  const auto tv = lds_twiddles<T>(lds + ntot, tw, nf1);
+  if (threadIdx.x < 64) walk_table_build(wt, off, g, p, row0, 1);
+  for (int i = threadIdx.x; i < ntot; i += nt) lds[i] = fftc::mk((T)0, (T)0);
+  __syncthreads();
+  const int total = wt.total;
+  // this part's image columns: grid_index(col) = col - N1/2 (mod nf1) = h (mod S)
+  const int N1 = g.N1, h1 = N1 / 2, c0 = (h + h1) % S, nout = (N1 - c0 + S - 1) / S;
+  const int lgt = 31 - __clz(g.nf2) + kTileLog2;
+  cpx<T> *hb = H + ((size_t)pl * g.nct * g.nf2 + row0) * kTile;
+  const int hs = h << g.slot_lg;  // this part's first column slot
+  if (total == 0) {
+    for (int i = threadIdx.x; i < nout; i += nt) {
+      const int sl = hs + i;
+      hb[((sl >> kTileLog2) << lgt) + (sl & (kTile - 1))] = fftc::mk((T)0, (T)0);
+    }
+    return;
+  }
+  const Ker<T> K = make_ker<T>(g);
+  for (int t0 = threadIdx.x; t0 < total; t0 += 2 * nt) {
+    const int t1 = t0 + nt;
+    const Rec<T> ra = recs[walk_record(wt, t0)];
+    Rec<T> rb;
+    if (t1 < total) rb = recs[walk_record(wt, t1)];
+    spread_row_part<T, KWM>(ra, ldsf, tv, g, p, row0, h, m - 1, sign, K);
+    if (t1 < total) spread_row_part<T, KWM>(rb, ldsf, tv, g, p, row0, h, m - 1, sign, K);
+  }
+  __syncthreads();
+  lds_fft_part<T, E, 16, S>(lds, 31 - __clz(m), sign, tv);
+  int tid = threadIdx.x;
+  __asm__ volatile("" : "+v"(tid));
+  constexpr int NO = E / 2;  // nout <= m / 2 = (E / 2) nt, since N1 <= nf1 / 2
+  cpx<T> o[NO];
+#pragma unroll
+  for (int e = 0; e < NO; ++e) {
+    const int i = tid + e * nt;
+    if (i < nout) {
+      const int col = S * i + c0;
+      const int k1 = col >= h1 ? col - h1 : nf1 + col - h1;
+      o[e] = lds[fftc::pad(k1 / S)];
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < NO; ++e) {
+    const int i = tid + e * nt;
+    if (i < nout) {
+      const int sl = hs + i;
+      hb[((sl >> kTileLog2) << lgt) + (sl & (kTile - 1))] = o[e];
+    }
+  }
+}
+
+// Interpolation of one record from part h of a row split S ways: the row's
+// cells are x[n] = sum_h w^(sign h n) Y_h[n mod m] (w = e^{2 pi i / nf1}, m = nf1 / S),
+// and part h holds Y_h; its share of the record's sum is added to the record's
+// FP64 accumulator.
+template <typename T, int KWM, typename Tw>
+__device__ __forceinline__ void interp_row_part(const Rec<T> &rc, int idx, const cpx<T> *lds, const Tw &tv,
+                                                const Geom &g, int p, int row0, int h, int mm, int sign,
+                                                double2 *__restrict__ vacc, const Ker<T> &K) {
+  const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2;
+  int k = p - rc.iw;
+  if (k < 0) k += g.P;
+  if (k >= g.kww) return;
+  int jj = row0 - rc.iy;
+  if (jj < 0) jj += nf2;
+  if (jj >= kw) return;
+  T rr = 0, ri = 0;
+  T kxs[KWM];
+  K.template taps<KWM>(rc.dx, kxs);
+#pragma unroll
+  for (int i = 0; i < KWM; ++i) {
+    if (i < kw) {
+      int cx = rc.ix + i;
+      if (cx >= nf1) cx -= nf1;
+      const T kx = kxs[i];
+      cpx<T> f = lds[fftc::pad(cx & mm)];
+      if (h) {
+        const cpx<T> w = tv[(h * cx) & (nf1 - 1)];
+        f = fftc::mulw(f, w.x, (T)sign * w.y);
+      }
+      rr += f.x * kx;
+      ri += f.y * kx;
+    }
+  }
+  const T kyz = K(rc.dy, jj) * K.w(rc.dw, k);
+  atomicAdd(&vacc[idx].x, (double)(rr * kyz));
+  atomicAdd(&vacc[idx].y, (double)(ri * kyz));
+}
+
+// dirty2ms row pass for rows too long for two workgroups per CU: each row is
+// split S ways by the index of its inverse-transform input, k = S k' + h
+// (radix-S decimation in frequency), one workgroup per part: Y_h = FFT_m over
+// X[S k' + h], so the parts of a row load disjoint columns of H, run m-point
+// FFTs and add their shares to each record's accumulator.
+template <typename T, int E, int KWM, int S>
+__global__ void __launch_bounds__(MaxThreads<T>::value / S) row_ifft_interp_part_kernel(
+    const cpx<T> *__restrict__ H, const Rec<T> *__restrict__ recs, const int *__restrict__ off,
+    double2 *__restrict__ vacc, const cpx<T> *__restrict__ tw, Geom g, int p0, int sign) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ WalkTable wt;
+  cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
+  const int nf1 = g.nf1, m = nf1 / S, N1 = g.N1, nt = blockDim.x;
+  const int item = xcd_group(blockIdx.x, gridDim.x);
+  const int row0 = item / S, h = item % S, pl = blockIdx.y, p = p0 + pl;
+  const int lgm = 31 - __clz(m), lgt = 31 - __clz(g.nf2) + kTileLog2;
+  const cpx<T> *hb = H + ((size_t)pl * g.nct * g.nf2 + row0) * kTile;
+  cpx<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = threadIdx.x + e * nt;  // < m = E * nt
+    const int col = image_index(S * i + h, N1, nf1), sl = (h << g.slot_lg) + col / S;
+    v[e] = col >= 0 ? hb[((sl >> kTileLog2) << lgt) + (sl & (kTile - 1))] : fftc::mk((T)0, (T)0);
+  }
+  const auto tv = lds_twiddles<T>(lds + fftc::padded_size(m), tw, nf1);
+  if (threadIdx.x < 64) walk_table_build(wt, off, g, p, row0, 1);
+#pragma unroll
+  for (int e = 0; e < E; ++e) lds[fftc::pad(threadIdx.x + e * nt)] = v[e];
+  __syncthreads();
+  const int total = wt.total;
+  if (total == 0) return;
+  int idx0 = 0;
+  Rec<T> r0;
+  if (threadIdx.x < total) {
+    idx0 = walk_record(wt, threadIdx.x);
+    r0 = recs[idx0];
+  }
+  lds_fft_part<T, E, 16, S>(lds, lgm, sign, tv);
+  const Ker<T> K = make_ker<T>(g);
+  if (threadIdx.x < total)
+    interp_row_part<T, KWM>(r0, idx0, lds, tv, g, p, row0, h, m - 1, sign, vacc, K);
+  for (int t = threadIdx.x + nt; t < total; t += nt) {
+    const int idx = walk_record(wt, t);
+    interp_row_part<T, KWM>(recs[idx], idx, lds, tv, g, p, row0, h, m - 1, sign, vacc, K);
+  }
+}
+
+// z' (n - 1 - D) gamma h of image pixel (row, col), cuft.cu:53-92 pre-stage
+__device__ __forceinline__ double pixel_zp(int row, int col, const Geom &g) {
+  double nm1, n;
+  pixel_n(row, col, g, nm1, n);
+  return (nm1 - g.D) * g.gamma * g.h;
+}
+
+// ms2dirty column pass fused with the w-accumulation (w_term_dft, cuft.cu:520-572):
+// a workgroup owns cpw grid columns (cpw * nf2 = E * blockDim.x) and walks the
+// batch's nb planes: load the row-transformed columns, FFT over nf2, then for
+// the N2 image rows accT[col][row] += F_p e^{+i z' (p - P/2)}: the phase
+// argument and its reduction in FP64 (sin / cos of the reduced argument at the
+// planes' precision) and the sum over planes in FP64 registers.  Image rows per thread:
+// cpw * N2 <= 8 * blockDim.x (the plan requires 2 N2 <= nf2).
+template <typename T, int E>
+__global__ void __launch_bounds__(MaxThreads<T>::value) col_fwd_kernel(const cpx<T> *__restrict__ H,
+                                                                      double2 *__restrict__ accT,
+                                                                      const cpx<T> *__restrict__ tw, Geom g,
+                                                                      int p0, int nb, int first) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
+  const int cpw = g.cpw, nf2 = g.nf2, N2 = g.N2, nt = blockDim.x;
+  const int s0 = xcd_group(blockIdx.x, gridDim.x) * cpw;  // first column slot
+  if (!slots_used(s0, g)) return;
+  const int lg = 31 - __clz(nf2), lgt = lg + kTileLog2;
+  const int nout = cpw * N2;
+  double ar[8], ai[8], zp[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    ar[e] = ai[e] = zp[e] = 0.0;
+    const int o = threadIdx.x + e * nt;
+    if (o < nout) {
+      const int ci = o / N2, row = o - ci * N2, col = slot_column(s0 + ci, g);
+      if (col >= 0) zp[e] = pixel_zp(row, col, g);
+    }
+  }
+  const auto tv = lds_twiddles<T>(lds + fftc::padded_size(cpw * nf2), tw, nf2);
+  for (int k = 0; k < nb; ++k) {
+    // opaque copy of the thread id: the per-plane addresses are formed inside
+    // the loop rather than held in registers across it
+    int tid = threadIdx.x;
+    __asm__ volatile("" : "+v"(tid));
+    const cpx<T> *hk = H + (size_t)k * g.nct * nf2 * kTile;
+    cpx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = tid + e * nt;
+      const int ci = i >> lg, k2 = i & (nf2 - 1), sl = s0 + ci;
+      v[e] = slot_column(sl, g) >= 0 ? hk[((sl >> kTileLog2) << lgt) + (k2 << kTileLog2) + (sl & (kTile - 1))]
+                                     : fftc::mk((T)0, (T)0);
+    }
+    __syncthreads();  // the previous plane's LDS reads are done
+#pragma unroll
+    for (int e = 0; e < E; ++e) lds[fftc::pad(tid + e * nt)] = v[e];
+    __syncthreads();
+    lds_fft_rows<T, E, 16>(lds, lg, cpw, +1, tv);
+    const double m = (double)(p0 + k - g.P / 2);
+    int tid2 = threadIdx.x;
+    __asm__ volatile("" : "+v"(tid2));
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int o = tid2 + e * nt;
+      if (o < nout) {
+        const int ci = o / N2, row = o - ci * N2;
+        const cpx<T> f = lds[fftc::pad(ci * nf2 + grid_index(row, N2, nf2))];
+        double sn, cs;
+        plane_phase<T>(zp[e] * m, &sn, &cs);
+        const double fr = (double)f.x, fi = (double)f.y;
+        ar[e] += fr * cs - fi * sn;
+        ai[e] += fr * sn + fi * cs;
+      }
+      // one element's phase at a time (bounds the live FP64 temporaries)
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int o = threadIdx.x + e * nt;
+    if (o < nout) {
+      const int ci = o / N2, row = o - ci * N2, col = slot_column(s0 + ci, g);
+      if (col >= 0) {
+        const size_t a = (size_t)col * N2 + row;
+        double2 r = make_double2(ar[e], ai[e]);
+        if (!first) {
+          const double2 q = accT[a];
+          r.x += q.x;
+          r.y += q.y;
+        }
+        accT[a] = r;
+      }
+    }
+  }
+}
+
+// dirty2ms column pass fused with the w-fill (w_term_idft, cuft.cu:575-612): a
+// workgroup owns cpw grid columns and, for each of the batch's nb planes, places
+// gT[col][row] e^{-i z' (p - P/2)} (FP64 phase argument) at the image rows of its columns
+// (zeros elsewhere), runs the inverse FFT over nf2 and writes the full columns
+// to the column-tiled H.
+template <typename T, int E>
+__global__ void __launch_bounds__(MaxThreads<T>::value) col_inv_kernel(const cpx<T> *__restrict__ gT,
+                                                                      cpx<T> *__restrict__ H,
+                                                                      const cpx<T> *__restrict__ tw, Geom g,
+                                                                      int p0, int nb) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
+  const int cpw = g.cpw, nf2 = g.nf2, N2 = g.N2, nt = blockDim.x;
+  const int s0 = xcd_group(blockIdx.x, gridDim.x) * cpw;  // first column slot
+  if (!slots_used(s0, g)) return;
+  const int lg = 31 - __clz(nf2);
+  const int nout = cpw * N2, nzero = cpw * (nf2 - N2), h2 = N2 / 2;
+  cpx<T> gv[8];
+  double zp[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    gv[e] = fftc::mk((T)0, (T)0);
+    zp[e] = 0.0;
+    const int o = threadIdx.x + e * nt;
+    if (o < nout) {
+      const int ci = o / N2, row = o - ci * N2, col = slot_column(s0 + ci, g);
+      if (col >= 0) {
+        gv[e] = gT[(size_t)col * N2 + row];
+        zp[e] = pixel_zp(row, col, g);
+      }
+    }
+  }
+  const auto tv = lds_twiddles<T>(lds + fftc::padded_size(cpw * nf2), tw, nf2);
+  for (int k = 0; k < nb; ++k) {
+    const double m = (double)(p0 + k - g.P / 2);
+    int tid = threadIdx.x;
+    __asm__ volatile("" : "+v"(tid));
+    __syncthreads();  // the previous plane's LDS reads are done
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int o = tid + e * nt;
+      if (o < nout) {
+        const int ci = o / N2, row = o - ci * N2;
+        double sn, cs;
+        plane_phase<T>(-zp[e] * m, &sn, &cs);
+        const double gr = (double)gv[e].x, gi = (double)gv[e].y;
+        lds[fftc::pad(ci * nf2 + grid_index(row, N2, nf2))] =
+            fftc::mk((T)(gr * cs - gi * sn), (T)(gr * sn + gi * cs));
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    for (int z = tid; z < nzero; z += nt) {
+      const int ci = z / (nf2 - N2), j = z - ci * (nf2 - N2);
+      lds[fftc::pad(ci * nf2 + N2 - h2 + j)] = fftc::mk((T)0, (T)0);
+    }
+    __syncthreads();
+    lds_fft_rows<T, E, 16>(lds, lg, cpw, -1, tv);
+    int tid2 = threadIdx.x;
+    __asm__ volatile("" : "+v"(tid2));
+    cpx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = lds[fftc::pad(tid2 + e * nt)];
+    cpx<T> *hk = H + (size_t)k * g.nct * nf2 * kTile;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = tid2 + e * nt;
+      const int ci = i >> lg, k2 = i & (nf2 - 1), sl = s0 + ci;
+      if (slot_column(sl, g) >= 0) hk[((sl >> kTileLog2) << (lg + kTileLog2)) + (k2 << kTileLog2) + (sl & (kTile - 1))] = v[e];
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// launchers (the LDS path takes kernel widths up to 16: the plan sends wider
+// kernels to the full-plane path)
+// --------------------------------------------------------------------------
+template <typename K>
+int allow_lds(K kernel, int bytes) {
+  if (bytes > 65536) CURIG_HIP(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  return kOk;
+}
+
+inline int col_groups(const Geom &g) { return (g.nslot + g.cpw - 1) / g.cpw; }
+
+// ms2dirty, one batch on the LDS path: spread + row FFT -> H (column-tiled)
+template <typename T>
+void lds_spread_rows(const Rec<T> *recs, const int *off, const Geom &g, int p0, int nb, cpx<T> *H,
+                     const cpx<T> *tw1, hipStream_t st) {
+  if (g.rsplit == kRowSplit) {
+    const dim3 grd(kRowSplit * g.nf2, nb);
+    const int nt = g.nf1 / (16 * kRowSplit), lb = g.lds_rowp_bytes;
+    if (g.kw <= 8)
+      spread_row_fft_part_kernel<T, 16, 8, kRowSplit><<<grd, nt, lb, st>>>(recs, off, H, tw1, g, p0);
+    else
+      spread_row_fft_part_kernel<T, 16, 16, kRowSplit><<<grd, nt, lb, st>>>(recs, off, H, tw1, g, p0);
+    return;
+  }
+  const dim3 grd(g.nf2 / g.rpw, nb);
+  const int nt = g.rpw * g.nf1 / 16;
+  if (g.kw <= 8)
+    spread_row_fft_kernel<T, 16, 8><<<grd, nt, g.lds_row_bytes, st>>>(recs, off, H, tw1, g, p0);
+  else
+    spread_row_fft_kernel<T, 16, 16><<<grd, nt, g.lds_row_bytes, st>>>(recs, off, H, tw1, g, p0);
+}
+
+template <typename T>
+int allow_spread_rows(const Geom &g) {
+  CURIG_TRY(allow_lds(spread_row_fft_kernel<T, 16, 8>, g.lds_row_bytes));
+  CURIG_TRY(allow_lds(spread_row_fft_kernel<T, 16, 16>, g.lds_row_bytes));
+  if (g.rsplit == kRowSplit) {
+    CURIG_TRY((allow_lds(spread_row_fft_part_kernel<T, 16, 8, kRowSplit>, g.lds_rowp_bytes)));
+    CURIG_TRY((allow_lds(spread_row_fft_part_kernel<T, 16, 16, kRowSplit>, g.lds_rowp_bytes)));
+  }
+  return kOk;
+}
+
+// dirty2ms, one batch on the LDS path: inverse row FFT fused with the interpolation
+template <typename T>
+void lds_row_interp(const Rec<T> *recs, const int *off, const cpx<T> *H, double2 *vacc, const cpx<T> *tw1,
+                    const Geom &g, int p0, int nb, hipStream_t st) {
+  if (g.rsplit == kRowSplit) {
+    const dim3 grd(kRowSplit * g.nf2, nb);
+    const int nt = g.nf1 / (16 * kRowSplit), lb = g.lds_rowp_bytes;
+    if (g.kw <= 8)
+      row_ifft_interp_part_kernel<T, 16, 8, kRowSplit><<<grd, nt, lb, st>>>(H, recs, off, vacc, tw1, g, p0, -1);
+    else
+      row_ifft_interp_part_kernel<T, 16, 16, kRowSplit><<<grd, nt, lb, st>>>(H, recs, off, vacc, tw1, g, p0, -1);
+    return;
+  }
+  const dim3 grd(g.nf2 / g.rpw, nb);
+  const int nt = g.rpw * g.nf1 / 16;
+  if (g.kw <= 8)
+    row_ifft_interp_kernel<T, 16, 8><<<grd, nt, g.lds_row_bytes, st>>>(H, recs, off, vacc, tw1, g, p0, -1);
+  else
+    row_ifft_interp_kernel<T, 16, 16><<<grd, nt, g.lds_row_bytes, st>>>(H, recs, off, vacc, tw1, g, p0, -1);
+}
+
+template <typename T>
+int allow_row_interp(const Geom &g) {
+  CURIG_TRY(allow_lds(row_ifft_interp_kernel<T, 16, 8>, g.lds_row_bytes));
+  CURIG_TRY(allow_lds(row_ifft_interp_kernel<T, 16, 16>, g.lds_row_bytes));
+  if (g.rsplit == kRowSplit) {
+    CURIG_TRY((allow_lds(row_ifft_interp_part_kernel<T, 16, 8, kRowSplit>, g.lds_rowp_bytes)));
+    CURIG_TRY((allow_lds(row_ifft_interp_part_kernel<T, 16, 16, kRowSplit>, g.lds_rowp_bytes)));
+  }
+  return kOk;
+}
+
+// ms2dirty, one batch on the LDS path: column FFT + w-accumulation into accT
+template <typename T>
+void lds_cols_forward(const Geom &g, int p0, int nb, const cpx<T> *H, double2 *accT, const cpx<T> *tw2, int first,
+                      hipStream_t st) {
+  col_fwd_kernel<T, 16><<<col_groups(g), g.cpw * g.nf2 / 16, g.lds_col_bytes, st>>>(H, accT, tw2, g, p0, nb, first);
+}
+
+// dirty2ms, one batch on the LDS path: w-fill + inverse column FFT -> H (column-tiled)
+template <typename T>
+void lds_cols_backward(const Geom &g, int p0, int nb, const cpx<T> *gT, cpx<T> *H, const cpx<T> *tw2,
+                       hipStream_t st) {
+  col_inv_kernel<T, 16><<<col_groups(g), g.cpw * g.nf2 / 16, g.lds_col_bytes, st>>>(gT, H, tw2, g, p0, nb);
+}
+
+template <typename T>
+int allow_cols(const Geom &g) {
+  CURIG_TRY(allow_lds(col_fwd_kernel<T, 16>, g.lds_col_bytes));
+  CURIG_TRY(allow_lds(col_inv_kernel<T, 16>, g.lds_col_bytes));
+  return kOk;
+}
+
+// one source per (precision, kernel group), so the heavy kernels compile in parallel
+#define CURIG_LDS_SPREAD_INST(T)                                                                                   \
+  template void lds_spread_rows<T>(const Rec<T> *, const int *, const Geom &, int, int, cpx<T> *, const cpx<T> *,  \
+                                   hipStream_t);                                                                   \
+  template int allow_spread_rows<T>(const Geom &);
+#define CURIG_LDS_INTERP_INST(T)                                                                                   \
+  template void lds_row_interp<T>(const Rec<T> *, const int *, const cpx<T> *, double2 *, const cpx<T> *,          \
+                                  const Geom &, int, int, hipStream_t);                                            \
+  template int allow_row_interp<T>(const Geom &);
+#define CURIG_LDS_COLS_INST(T)                                                                                     \
+  template void lds_cols_forward<T>(const Geom &, int, int, const cpx<T> *, double2 *, const cpx<T> *, int,        \
+                                    hipStream_t);                                                                  \
+  template void lds_cols_backward<T>(const Geom &, int, int, const cpx<T> *, cpx<T> *, const cpx<T> *,             \
+                                     hipStream_t);                                                                 \
+  template int allow_cols<T>(const Geom &);
+
+}  // namespace curig

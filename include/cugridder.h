@@ -39,7 +39,12 @@ typedef std::complex<double> curig_c128;
  * (metres) and d_c are owned by the plan after gridder_setting; set plan->fk to
  * the device image before gridder_execution.  gridder_destroy frees the plan's
  * device buffers, the host arrays in gridder_plan->kv and both plan structs
- * (allocate them with malloc / new of the POD struct), as the reference does. */
+ * (allocate them with malloc / new of the POD struct), as the reference does.
+ * kerevalmeth: 0 exp(sqrt()) kernel values, 1 Horner's rule on a piecewise
+ * polynomial.  plan->opts.precision (set before gridder_setting; 0 after a
+ * memset) selects the data type of d_c and plan->fk: 0 complex128 (the
+ * reference's CUCPX), 1 complex64 (FP32-mixed); kv.weight stays float64 on the
+ * host either way. */
 int gridder_setting(int N1, int N2, int method, int kerevalmeth, int w_term_method, PCS tol, int direction,
                     double sigma, int iflag, int batchsize, int M, int channel, PCS fov, visibility *pointer_v,
                     PCS *d_u, PCS *d_v, PCS *d_w, CUCPX *d_c, curafft_plan *plan, ragridder_plan *gridder_plan);

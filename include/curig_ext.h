@@ -78,6 +78,14 @@ int curig_explicit_ms2dirty_dev(int nrow, int nxdirty, int nydirty, double fov, 
                                 const double *d_uvw, const void *d_vis, const void *d_wgt, int precision,
                                 long npix, const int64_t *d_pix, void *d_out, int sign, void *stream);
 
+/* FP64 direct sum of the adjoint (dirty2ms) on the GPU at nsel chosen rows:
+ * out[q] (complex128) = wgt_j sum_{i,k} dirty[i][k] / n_ik
+ * e^{-2 pi i f/c (u_j l_i + v_j m_k + sign w_j (n_ik - 1))}, j = d_rows[q].
+ * dirty / wgt precision as `precision`. */
+int curig_explicit_dirty2ms_dev(int nrow, int nxdirty, int nydirty, double fov, double freq,
+                                const double *d_uvw, const void *d_dirty, const void *d_wgt, int precision,
+                                long nsel, const int64_t *d_rows, void *d_out, int sign, void *stream);
+
 /* Generic NUFFT on device pointers (the work behind cunufft_exec, declared in
  * cufft_nufft.h): type 1 (c -> fk) or 2 (fk -> c), dimension 1-3, modes
  * N1 (fast) x N2 x N3 (unused dimensions ignored), M points with coordinates

@@ -31,6 +31,10 @@
 #include <type_traits>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
+#include <cstdlib>
+
 #include "curig_internal.h"
 #include "fft.h"
 #include "fft_core.h"
@@ -149,6 +153,37 @@ struct Timer {
   }
   ~Timer() {
     for (auto e : ev) (void)hipEventDestroy(e);
+  }
+};
+
+// Per-stage tracing, switched on by environment variables read once:
+//   CURIG_TRACE=1   roctx ranges around each call and each of its stages
+//                   (binning, every plane batch, epilogue), seen by
+//                   rocprofv3 --marker-trace;
+//   CURIG_TIMING=1  every call times its stages with HIP events (as the
+//                   stats API does) and prints them to stderr.
+struct TraceFlags {
+  bool ranges = false, timing = false;
+  TraceFlags() {
+    const char *t = std::getenv("CURIG_TRACE");
+    const char *m = std::getenv("CURIG_TIMING");
+    ranges = t && *t && *t != '0';
+    timing = m && *m && *m != '0';
+  }
+};
+const TraceFlags &trace_flags() {
+  static const TraceFlags f;
+  return f;
+}
+
+// roctx range for the enclosing scope (no-op unless CURIG_TRACE is set)
+struct Range {
+  bool on;
+  explicit Range(const char *name) : on(trace_flags().ranges) {
+    if (on) roctxRangePushA(name);
+  }
+  ~Range() {
+    if (on) roctxRangePop();
   }
 };
 
@@ -633,7 +668,10 @@ int enqueue_ms2dirty(const Problem &pb, const ChanPlan &cp, const Bufs &b, hipSt
   tm.on = sts != nullptr;
   tm.s = st;
   int e0 = tm.mark();
-  enqueue_binning<T>(pb, cp, b, st, true);
+  {
+    Range r("curig ms2dirty binning");
+    enqueue_binning<T>(pb, cp, b, st, true);
+  }
   CURIG_HIP(hipGetLastError());
   int e1 = tm.mark();
   const long npix = (long)g.N1 * g.N2;
@@ -643,6 +681,7 @@ int enqueue_ms2dirty(const Problem &pb, const ChanPlan &cp, const Bufs &b, hipSt
   double t_spread = 0, t_fft = 0, t_wacc = 0;
   for (int bi = 0; bi < g.nbatch; ++bi) {
     const int p0 = bi * g.Pb, nb = std::min(g.Pb, g.P - p0);
+    Range r("curig ms2dirty plane batch");
     const int a = tm.mark();
     if (g.fft_lds) {
       // spread + row FFT counts as spread (t_grid); the column FFT fused with
@@ -670,6 +709,7 @@ int enqueue_ms2dirty(const Problem &pb, const ChanPlan &cp, const Bufs &b, hipSt
     tm.span(c, d, &t_wacc);
   }
   int e2 = tm.mark();
+  Range r("curig ms2dirty epilogue");
   launch_epilogue(pb.dbl, b.acc, pb.dirty, g, fk1, fk2, q, pb.accumulate ? 1 : 0, g.fft_lds, st);
   CURIG_HIP(hipGetLastError());
   int e3 = tm.mark();
@@ -695,6 +735,7 @@ int enqueue_dirty2ms(const Problem &pb, const ChanPlan &cp, const Bufs &b, hipSt
   tm.on = sts != nullptr;
   tm.s = st;
   int e0 = tm.mark();
+  Range rp("curig dirty2ms binning and prologue");
   enqueue_binning<T>(pb, cp, b, st, false);
   // every output row is written: flagged rows get 0
   launch_zero_rows(pb.dbl, pb.vis, pb.vis_stride, pb.nrow, st);
@@ -714,6 +755,7 @@ int enqueue_dirty2ms(const Problem &pb, const ChanPlan &cp, const Bufs &b, hipSt
   double t_interp = 0, t_fft = 0, t_fill = 0;
   for (int bi = 0; bi < g.nbatch; ++bi) {
     const int p0 = bi * g.Pb, nb = std::min(g.Pb, g.P - p0);
+    Range r("curig dirty2ms plane batch");
     const int a = tm.mark();
     if (g.fft_lds) {
       // w-fill + column iFFT counts as FFT (t_fft); the row iFFT fused with
@@ -743,6 +785,7 @@ int enqueue_dirty2ms(const Problem &pb, const ChanPlan &cp, const Bufs &b, hipSt
     tm.span(c, d, &t_interp);
   }
   int e2 = tm.mark();
+  Range rf("curig dirty2ms finalize");
   launch_finalize(pb.dbl, b.perm, b.vacc, (int)cp.nvalid, pb.uvw, pb.wgt, pb.wgt_stride, pb.vis, pb.vis_stride, g,
                   st);
   CURIG_HIP(hipGetLastError());
@@ -911,7 +954,18 @@ int run_problems(bool to_image, const Problem *pbs, int n, hipStream_t st, const
   CURIG_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lk(g_ws_mu);
   CURIG_TRY(ws_order(dev, st));
+  Range r(to_image ? "curig ms2dirty" : "curig dirty2ms");
+  Stats own;
+  const bool print = trace_flags().timing;
+  if (print && !sts) sts = &own;
   const int ier = run_locked<T>(to_image, pbs, n, st, tune, graph_mode, sts, dev);
+  if (print && ier == kOk)
+    fprintf(stderr,
+            "[curig] %s %s: %d channel(s), %ld vis, grid %d x %d x %d (%d batches), prep %.3f ms, %s %.3f ms, "
+            "fft %.3f ms, %s %.3f ms, final %.3f ms, total %.3f ms\n",
+            to_image ? "ms2dirty" : "dirty2ms", sizeof(T) == 8 ? "fp64" : "fp32", n, sts->nvis, sts->nf2, sts->nf1,
+            sts->P, sts->nbatch, sts->t_prep, to_image ? "spread" : "interp", sts->t_grid, sts->t_fft,
+            to_image ? "w-acc" : "w-fill", sts->t_plane, sts->t_final, sts->t_total);
   // marked on every path: a failed call may have queued launches too
   const int ierm = ws_mark(dev, st);
   return ier != kOk ? ier : ierm;

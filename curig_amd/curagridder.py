@@ -1,6 +1,7 @@
 """cuRIG-style Python API: vis2dirty / dirty2vis (sign +1).
 
-Mirrors python/curagridder/curagridder.py:187-246 of the reference: same names,
+Mirrors python/curagridder/curagridder.py:69-128 of the reference (vis2dirty
+:69-98, dirty2vis :100-128): same names,
 arguments and semantics (the caller provides the complex128 output array, which
 is filled in place and returned).  Additionally accepts precision="fp32" for the
 FP32-mixed form (complex64 data, float32 weights)."""
@@ -39,7 +40,7 @@ def vis2dirty(uvw, freq, ms, wgt, dirty, fov, epsilon=1e-6, sigma=1.25, precisio
     """Image from visibilities; fills and returns `dirty` ([nx][ny] complex).
 
     uvw [nrow,3] metres, freq (freq[0] used), ms [nrow] complex, wgt [nrow] or None,
-    fov degrees.  sign = +1 (python/curagridder/curagridder.py:205)."""
+    fov degrees.  sign = +1 (python/curagridder/curagridder.py:87, 121)."""
     cd = np.complex128 if precision == "fp64" else np.complex64
     _check_out(dirty, "dirty", cd)
     ms = np.ascontiguousarray(ms, dtype=cd).reshape(-1)

@@ -8,13 +8,14 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-SRC = os.path.join(ROOT, "curig_amd", "csrc", "fft_core_check.cpp")
+SRC = os.path.join(ROOT, "tests", "native", "fft_core_check.cpp")
+CSRC = os.path.join(ROOT, "curig_amd", "csrc")
 
 
 @pytest.fixture(scope="module")
 def lib(tmp_path_factory):
     out = str(tmp_path_factory.mktemp("fftc") / "libfftc.so")
-    subprocess.check_call(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-o", out, SRC])
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-I", CSRC, "-o", out, SRC])
     L = ctypes.cdll.LoadLibrary(out)
     L.fftc_check_run.argtypes = [ctypes.c_int] * 7 + [np.ctypeslib.ndpointer(np.float64, flags="C"), ctypes.c_int,
                                                       ctypes.c_int, ctypes.c_int]

@@ -9,13 +9,14 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-SRC = os.path.join(ROOT, "curig_amd", "csrc", "trig_check.cpp")
+SRC = os.path.join(ROOT, "tests", "native", "trig_check.cpp")
+CSRC = os.path.join(ROOT, "curig_amd", "csrc")
 
 
 @pytest.fixture(scope="module")
 def lib(tmp_path_factory):
     out = str(tmp_path_factory.mktemp("trig") / "libtrig.so")
-    subprocess.check_call(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-o", out, SRC])
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-I", CSRC, "-o", out, SRC])
     L = ctypes.cdll.LoadLibrary(out)
     f64 = np.ctypeslib.ndpointer(np.float64, flags="C")
     L.trig_check_run.argtypes = [ctypes.c_long, f64, f64, f64]

@@ -19,7 +19,7 @@ _c128 = np.ctypeslib.ndpointer(np.complex128, flags="C")
 # every symbol include/cugridder.h and include/curig_ext.h declare for C callers
 EXPORTS = [
     "ms2dirty_1", "ms2dirty_2", "dirty2ms_1", "dirty2ms_2",
-    "ms2dirty_f32", "dirty2ms_f32", "curig_ms2dirty_dev", "curig_dirty2ms_dev",
+    "ms2dirty_f32", "dirty2ms_f32", "curig_ms2dirty_dev", "curig_dirty2ms_dev", "curig_ms2dirty_dev_noalloc",
     "curig_ms2dirty_multi_dev", "curig_dirty2ms_multi_dev", "curig_explicit_ms2dirty_dev",
     "curig_explicit_dirty2ms_dev",
     "curig_nifty_ms2dirty", "curig_nifty_dirty2ms", "curig_nufft_dev", "curig_nufft3_dev",
@@ -69,7 +69,7 @@ def load():
         f.restype = c_int
     dev = [c_int, c_int, c_int, c_double, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double,
            c_int, c_int, c_void_p]
-    for name in ("curig_ms2dirty_dev", "curig_dirty2ms_dev"):
+    for name in ("curig_ms2dirty_dev", "curig_dirty2ms_dev", "curig_ms2dirty_dev_noalloc"):
         f = getattr(lib, name)
         f.argtypes = dev
         f.restype = c_int

@@ -208,6 +208,16 @@ extern "C" int curig_ms2dirty_dev(int nrow, int nxdirty, int nydirty, double fov
   return ms2dirty_device(pb, (hipStream_t)stream, &g_tuning, stats_ptr());
 }
 
+extern "C" int curig_ms2dirty_dev_noalloc(int nrow, int nxdirty, int nydirty, double fov, double freq,
+                                          const double *d_uvw, const void *d_vis, const void *d_wgt, void *d_dirty,
+                                          double epsilon, double sigma, int sign, int precision, void *stream) {
+  CURIG_TRY(check_precision(precision));
+  CURIG_TRY(precheck(nrow, nxdirty, nydirty, epsilon, sigma));
+  Problem pb = dev_problem(nrow, nxdirty, nydirty, fov, freq, d_uvw, d_vis, d_wgt, d_dirty, epsilon, sigma, sign,
+                           precision);
+  return ms2dirty_device_noalloc(pb, (hipStream_t)stream, &g_tuning);
+}
+
 extern "C" int curig_dirty2ms_dev(int nrow, int nxdirty, int nydirty, double fov, double freq, const double *d_uvw,
                                   void *d_vis, const void *d_wgt, const void *d_dirty, double epsilon, double sigma,
                                   int sign, int precision, void *stream) {

@@ -53,6 +53,16 @@ struct Stats {
 int ms2dirty_device(const Problem &pb, hipStream_t stream, const Tuning *tune, Stats *st);
 int dirty2ms_device(const Problem &pb, hipStream_t stream, const Tuning *tune, Stats *st);
 
+// ms2dirty_device without any device allocation: every buffer is carved from
+// a fixed per-device scratch array of noalloc_scratch_bytes() (128 MiB), with
+// as many w planes per batch as fit; rejects (kErrScratch) plans off the LDS
+// FFT path and working sets beyond the scratch.  Calls share the scratch:
+// they are serialised on the host and ordered across streams like the other
+// entry points; a caller that races the library's queued work from its own
+// kernels on the output races as usual.
+int ms2dirty_device_noalloc(const Problem &pb, hipStream_t stream, const Tuning *tune);
+size_t noalloc_scratch_bytes();
+
 // Several problems (channels) sharing image size, epsilon and sigma, queued in
 // order on one stream.  graph_mode != 0: the whole sequence is captured once into
 // a hipGraph and replayed while the same call repeats (LDS FFT path only; other

@@ -70,7 +70,49 @@ struct Workspace {
 std::mutex g_ws_mu;
 std::vector<Workspace> g_ws;
 
+// Fixed device scratch of the entry point that allocates nothing
+// (ms2dirty_device_noalloc): a file-scope array every device holds once the
+// library runs there, carved into the call's buffers instead of the workspace.
+// In host code `scratch` is only a stand-in: its device address comes from
+// hipGetSymbolAddress, once per device (scratch_base).
+constexpr size_t kScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+struct Arena {
+  char *base = nullptr;
+  size_t cap = 0, used = 0;
+};
+Arena *g_arena = nullptr;        // set while a no-allocation call runs (g_ws_mu held)
+std::vector<char *> g_scratch;   // per device: the scratch array's device address
+
+int scratch_base(int dev, char **out) {
+  if ((int)g_scratch.size() <= dev) g_scratch.resize(dev + 1, nullptr);
+  if (!g_scratch[dev]) {
+    void *ptr = nullptr;
+    CURIG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) {
+      set_error("no device address for the scratch array");
+      return kErrHip;
+    }
+    g_scratch[dev] = (char *)ptr;
+  }
+  *out = g_scratch[dev];
+  return kOk;
+}
+
 int ws_get(int dev, Slot s, size_t bytes, void **out) {
+  if (g_arena) {
+    // no-allocation call: consecutive 256-byte aligned pieces of the scratch
+    const size_t a = ((bytes ? bytes : 16) + 255) & ~size_t(255);
+    if (g_arena->used + a > g_arena->cap) {
+      set_error("working set exceeds the %zu MiB fixed scratch (needs %zu more bytes at buffer %d)",
+                g_arena->cap >> 20, g_arena->used + a - g_arena->cap, (int)s);
+      return kErrScratch;
+    }
+    *out = g_arena->base + g_arena->used;
+    g_arena->used += a;
+    return kOk;
+  }
   if ((int)g_ws.size() <= dev) g_ws.resize(dev + 1);
   Workspace &w = g_ws[dev];
   if (bytes == 0) bytes = 16;
@@ -485,7 +527,8 @@ int twiddles(int dev, const Geom &g, hipStream_t st, const cpx<T> *&tw1, const c
   CURIG_TRY(ws_get(dev, S_TW, bytes, &vp));
   cpx<T> *t = (cpx<T> *)vp;
   int *key = g_ws[dev].tw_key;
-  if (key[0] != g.nf1 || key[1] != g.nf2 || key[2] != (int)sizeof(T)) {
+  // the scratch of a no-allocation call holds whatever the last call left there
+  if (g_arena || key[0] != g.nf1 || key[1] != g.nf2 || key[2] != (int)sizeof(T)) {
     std::vector<cpx<T>> h(n1 + n2);
     auto fill = [](cpx<T> *o, int n) {
       const int lb = fftc::twiddle_split(n), nhi = n >> lb, nlo = 1 << lb;
@@ -499,9 +542,11 @@ int twiddles(int dev, const Geom &g, hipStream_t st, const cpx<T> *&tw1, const c
     fill(h.data() + n1, g.nf2);
     CURIG_HIP(hipMemcpyAsync(t, h.data(), bytes, hipMemcpyHostToDevice, st));
     CURIG_HIP(hipStreamSynchronize(st));
-    key[0] = g.nf1;
-    key[1] = g.nf2;
-    key[2] = (int)sizeof(T);
+    if (!g_arena) {
+      key[0] = g.nf1;
+      key[1] = g.nf2;
+      key[2] = (int)sizeof(T);
+    }
   }
   tw1 = t;
   tw2 = t + n1;
@@ -873,7 +918,31 @@ int run_locked(bool to_image, const Problem *pbs, int n, hipStream_t st, const T
   std::vector<ChanPlan> cps;
   CURIG_TRY(plan_all(pbs, n, st, tune, dev, cps));
   Bufs b;
-  CURIG_TRY((reserve<T>(cps, pbs, to_image, dev, st, b)));
+  if (g_arena) {
+    // no-allocation call: the LDS FFT path only (rocFFT plans allocate), and
+    // as many planes per batch as the scratch holds
+    for (const ChanPlan &cp : cps)
+      if (!cp.g.fft_lds) {
+        set_error("the no-allocation entry needs the LDS FFT path (power-of-two grid of at most 16384 per axis, "
+                  "image at most half the grid, kernel width <= 16); grid %d x %d, kernel width %d",
+                  cp.g.nf2, cp.g.nf1, cp.g.kw);
+        return kErrScratch;
+      }
+    const size_t used0 = g_arena->used;
+    int ier = kErrScratch;
+    for (int Pb = cps[0].g.Pb; Pb >= 1; Pb = Pb > 1 ? Pb / 2 : 0) {
+      for (ChanPlan &cp : cps) {
+        cp.g.Pb = std::min(Pb, cp.g.P);
+        cp.g.nbatch = (cp.g.P + cp.g.Pb - 1) / cp.g.Pb;
+      }
+      g_arena->used = used0;
+      ier = reserve<T>(cps, pbs, to_image, dev, st, b);
+      if (ier != kErrScratch) break;
+    }
+    CURIG_TRY(ier);
+  } else {
+    CURIG_TRY((reserve<T>(cps, pbs, to_image, dev, st, b)));
+  }
   for (ChanPlan &cp : cps) {
     cp.g.hcoef_d = b.hc_d;
     cp.g.hcoef_f = b.hc_f;
@@ -982,6 +1051,27 @@ int dirty2ms_device(const Problem &pb, hipStream_t st, const Tuning *tune, Stats
   return pb.dbl ? run_problems<double>(false, &pb, 1, st, tune, 0, sts)
                 : run_problems<float>(false, &pb, 1, st, tune, 0, sts);
 }
+
+int ms2dirty_device_noalloc(const Problem &pb, hipStream_t st, const Tuning *tune) {
+  CURIG_TRY(check_args(pb));
+  int dev = 0;
+  CURIG_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  CURIG_TRY(ws_order(dev, st));
+  char *base = nullptr;
+  CURIG_TRY(scratch_base(dev, &base));
+  Arena ar;
+  ar.base = base;
+  ar.cap = kScratchBytes;
+  g_arena = &ar;
+  const int ier = pb.dbl ? run_locked<double>(true, &pb, 1, st, tune, 0, nullptr, dev)
+                         : run_locked<float>(true, &pb, 1, st, tune, 0, nullptr, dev);
+  g_arena = nullptr;
+  const int ierm = ws_mark(dev, st);
+  return ier != kOk ? ier : ierm;
+}
+
+size_t noalloc_scratch_bytes() { return kScratchBytes; }
 
 int multi_device(bool to_image, const Problem *pbs, int n, hipStream_t st, const Tuning *tune, int graph_mode,
                  Stats *sts) {

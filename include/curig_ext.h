@@ -35,6 +35,21 @@ int curig_dirty2ms_dev(int nrow, int nxdirty, int nydirty, double fov, double fr
                        void *d_vis, const void *d_wgt, const void *d_dirty, double epsilon, double sigma, int sign,
                        int precision, void *stream);
 
+/* curig_ms2dirty_dev without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+ * working buffer comes from a fixed 128 MiB scratch array that each device
+ * holds once the library has run there; the call takes as many w planes per
+ * batch as fit.  Returns 10 (with curig_last_error) when the working set
+ * exceeds the scratch or the plan is off the LDS FFT path (power-of-two grid of
+ * at most 16384 per axis, image at most half the grid, kernel width <= 16).
+ * Calls share the scratch: the library serialises them on the host and orders
+ * calls on different streams after each other; the output is complete once
+ * `stream` reaches the call's end. */
+int curig_ms2dirty_dev_noalloc(int nrow, int nxdirty, int nydirty, double fov, double freq, const double *d_uvw,
+                               const void *d_vis, const void *d_wgt, void *d_dirty, double epsilon, double sigma,
+                               int sign, int precision, void *stream);
+
 /* Device-resident multi-channel.  freq: host array [nchan].  d_flag: uint8
  * [nrow][nchan], nonzero excludes the sample (NULL: none).  ms2dirty writes
  * nchan images [nchan][nx][ny] when per_channel != 0, else one image summed over

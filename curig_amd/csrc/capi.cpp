@@ -733,7 +733,7 @@ extern "C" int curig_set_tuning(int tile_x, int tile_y, int planes_per_batch, in
 }
 
 extern "C" int curig_set_fft_path(int path) {
-  if (path != 0 && path != 1) {
+  if (path < 0 || path > 1) {
     set_error("fft path must be 0 (auto) or 1 (rocFFT)");
     return kErrArg;
   }

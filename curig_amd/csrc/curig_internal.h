@@ -99,6 +99,7 @@ struct Geom {
   int rpw;            // grid rows per workgroup in the row pass (rpw * nf1 = 16 * threads)
   int cpw;            // grid columns per workgroup in the column pass (cpw * nf2 = 16 * threads)
   int nct;            // column tiles of a plane's row-transformed layout (nslot / kTile, rounded up)
+  int tstride;        // elements from one column tile to the next: nf2 * kTile + a pad
   int nslot;          // column slots of that layout: N1, or nf1 / 2 with split rows
   int slot_lg;        // split rows: log2 of the slots per row part (nf1 / (2 rsplit))
   int lds_row_bytes;  // dynamic LDS of the row-pass workgroup
@@ -115,6 +116,13 @@ struct Geom {
 #define CURIG_TILE_LOG2 2
 #endif
 constexpr int kTileLog2 = CURIG_TILE_LOG2;
+// Bytes between consecutive column tiles beyond their nf2 * kTile elements.
+// Without it the tiles of a plane sit 2^k bytes apart and the kTile-column
+// chunks that one row writes (or reads) all fall into the same memory channel.
+#ifndef CURIG_TILE_PAD_BYTES
+#define CURIG_TILE_PAD_BYTES 4352
+#endif
+constexpr int kTilePadBytes = CURIG_TILE_PAD_BYTES;
 // Parts a long row is split into (2, 4 or 8; see the row-part kernels).  4
 // measured fastest at 8192^2 on MI355X (FP32 row pass 27.4 -> 25.1 ms and
 // interpolation 24.5 -> 20.5 ms per 32-plane launch against 2).

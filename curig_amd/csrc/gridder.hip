@@ -54,7 +54,7 @@ namespace {
 // previous call's waits for that call's launches on the device (ws_order).
 enum Slot {
   S_PART, S_CNT, S_HIST, S_OFF, S_BSUM, S_BINRANK, S_RECS, S_PERM, S_GRID, S_ACC, S_VACC, S_TAB, S_FFTW,
-  S_H, S_TW, S_HC,
+  S_H, S_TW, S_HC, S_TAPS,
   S_NSLOT
 };
 struct Workspace {
@@ -155,10 +155,6 @@ int ws_mark(int dev, hipStream_t st) {
   return kOk;
 }
 
-inline int imod_host(int a, int n) {
-  int r = a % n;
-  return r < 0 ? r + n : r;
-}
 
 
 struct Timer {
@@ -374,7 +370,8 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
   // pruned LDS FFT: power-of-two grids whose rows and columns fit one
   // workgroup's LDS, with the image at most half the grid on each axis
   g.fft_lds = 0;
-  g.rpw = g.cpw = g.nct = g.nslot = g.slot_lg = g.lds_row_bytes = g.lds_col_bytes = g.rsplit = g.lds_rowp_bytes = 0;
+  g.rpw = g.cpw = g.nct = g.tstride = g.nslot = g.slot_lg = g.lds_row_bytes = g.lds_col_bytes = g.rsplit =
+      g.lds_rowp_bytes = 0;
   auto pow2 = [](int n) { return n > 0 && (n & (n - 1)) == 0; };
   const size_t cs = dbl ? 16 : 8;
   const size_t lds_max = tn.lds_bytes > 0 ? (size_t)tn.lds_bytes : size_t(160) << 10;
@@ -403,12 +400,13 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
             (int)((size_t)(fftc::padded_size(g.nf1 / kRowSplit) + fftc::twiddle_entries(g.nf1)) * cs);
       }
       g.nct = (g.nslot + kTile - 1) / kTile;
+      g.tstride = g.nf2 * kTile + (int)(kTilePadBytes / cs);
     }
   }
   // planes per batch from the memory budget: full planes (rocFFT path) or, on
   // the LDS path, the compact plane plus the one or two column-pass buffers
   size_t per_plane = (size_t)g.nf1 * g.nf2 * cs;
-  if (g.fft_lds) per_plane = (size_t)g.nf2 * g.nct * kTile * cs;
+  if (g.fft_lds) per_plane = (size_t)g.nct * g.tstride * cs;
   int Pb = tn.Pb;
   if (Pb <= 0) {
     Pb = (int)std::max<size_t>(1, tn.grid_budget_bytes / per_plane);
@@ -593,7 +591,7 @@ int reserve(const std::vector<ChanPlan> &cps, const Problem *pbs, bool to_image,
     nvalid = std::max(nvalid, (size_t)std::max<long>(cps[i].nvalid, 1));
     const size_t plane = (size_t)g.nf1 * g.nf2, pix = (size_t)g.N1 * g.N2;
     if (!g.fft_lds) grid = std::max(grid, plane * g.Pb);
-    if (g.fft_lds) h = std::max(h, (size_t)g.nf2 * g.nct * kTile * g.Pb);
+    if (g.fft_lds) h = std::max(h, (size_t)g.nct * g.tstride * g.Pb);
     npix = std::max(npix, pix);
   }
   void *vp;
@@ -645,6 +643,8 @@ int reserve(const std::vector<ChanPlan> &cps, const Problem *pbs, bool to_image,
   if (g0.fft_lds) {
     CURIG_TRY(ws_get(dev, S_H, sizeof(cpx<T>) * h, &vp));
     b.H = vp;
+    CURIG_TRY(ws_get(dev, S_TAPS, sizeof(T) * 3 * (size_t)tap_width(g0.kw) * nrow, &vp));
+    b.taps = vp;
     const cpx<T> *t1, *t2;
     CURIG_TRY(twiddles<T>(dev, g0, st, t1, t2));
     b.tw1 = t1;
@@ -719,7 +719,6 @@ int enqueue_ms2dirty(const Problem &pb, const ChanPlan &cp, const Bufs &b, hipSt
   }
   CURIG_HIP(hipGetLastError());
   int e1 = tm.mark();
-  const long npix = (long)g.N1 * g.N2;
   const Rec<T> *recs = (const Rec<T> *)b.recs;
   v2_t<T> *grid = (v2_t<T> *)b.grid;
   const double *fk1 = b.tabs, *fk2 = b.tabs + (g.nf1 / 2 + 1), *q = fk2 + (g.nf2 / 2 + 1);
@@ -731,7 +730,7 @@ int enqueue_ms2dirty(const Problem &pb, const ChanPlan &cp, const Bufs &b, hipSt
     if (g.fft_lds) {
       // spread + row FFT counts as spread (t_grid); the column FFT fused with
       // the w-accumulation as FFT (t_fft)
-      lds_spread_rows<T>(recs, b.off, g, p0, nb, (cpx<T> *)b.H, (const cpx<T> *)b.tw1, st);
+      lds_spread_rows<T>(recs, (const T *)b.taps, b.off, g, p0, nb, (cpx<T> *)b.H, (const cpx<T> *)b.tw1, st);
       CURIG_HIP(hipGetLastError());
       const int bm = tm.mark();
       lds_cols_forward<T>(g, p0, nb, (const cpx<T> *)b.H, b.acc, (const cpx<T> *)b.tw2, bi == 0 ? 1 : 0, st);
@@ -784,8 +783,6 @@ int enqueue_dirty2ms(const Problem &pb, const ChanPlan &cp, const Bufs &b, hipSt
   enqueue_binning<T>(pb, cp, b, st, false);
   // every output row is written: flagged rows get 0
   launch_zero_rows(pb.dbl, pb.vis, pb.vis_stride, pb.nrow, st);
-  const long npix = (long)g.N1 * g.N2;
-  const size_t plane = (size_t)g.nf1 * g.nf2;
   const Rec<T> *recs = (const Rec<T> *)b.recs;
   v2_t<T> *grid = (v2_t<T> *)b.grid;
   double2 *gimg = b.acc;
@@ -809,7 +806,8 @@ int enqueue_dirty2ms(const Problem &pb, const ChanPlan &cp, const Bufs &b, hipSt
       CURIG_HIP(hipGetLastError());
       const int c = tm.mark();
       if (cp.nvalid > 0)
-        lds_row_interp<T>(recs, b.off, (const cpx<T> *)b.H, b.vacc, (const cpx<T> *)b.tw1, g, p0, nb, st);
+        lds_row_interp<T>(recs, (const T *)b.taps, b.off, (const cpx<T> *)b.H, b.vacc, (const cpx<T> *)b.tw1, g, p0,
+                          nb, st);
       CURIG_HIP(hipGetLastError());
       const int d = tm.mark();
       tm.span(a, c, &t_fft);
@@ -894,10 +892,10 @@ uint64_t graph_key(bool to_image, const Problem *pbs, int n, const std::vector<C
     k << cp.nvalid << g.N1 << g.N2 << g.nf1 << g.nf2 << g.P << g.kw << g.kww << g.hdeg << g.hcoef_d << g.hcoef_f
       << g.beta << g.c << g.dx << g.dy << g.divn << g.zout << g.C << g.D << g.gamma << g.h << g.uscale << g.vscale
       << g.wscale << g.T1 << g.T2 << g.ntx << g.nty << g.ntiles << g.Pb << g.nbatch << g.Pbin << g.nlayer
-      << g.nquad << g.fft_lds << g.rpw << g.cpw << g.nct << g.nslot << g.slot_lg << g.lds_row_bytes
+      << g.nquad << g.fft_lds << g.rpw << g.cpw << g.nct << g.tstride << g.nslot << g.slot_lg << g.lds_row_bytes
       << g.lds_col_bytes << g.rsplit << g.lds_rowp_bytes;
   }
-  k << b.dev << b.tabs << b.hist << b.off << b.bsum << b.perm << b.binrank << b.recs << b.grid << b.H << b.acc
+  k << b.dev << b.tabs << b.hist << b.off << b.bsum << b.perm << b.binrank << b.recs << b.taps << b.grid << b.H << b.acc
     << b.vacc << b.tw1 << b.tw2 << b.hc_d << b.hc_f;
   return k.h;
 }

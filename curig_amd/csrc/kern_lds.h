@@ -187,10 +187,43 @@ __device__ __forceinline__ int grid_index(int i, int N, int nf) {
   return i >= h ? i - h : nf + i - h;
 }
 
+// A record's kernel taps, evaluated once when the rows were binned
+// (bin_scatter_kernel): x along nf1 (KWM cells from ix), y along nf2, z along
+// the w planes; zero beyond the kernel width.
+template <typename T, int KWM> struct Taps {
+  T x[KWM], y[KWM], z[KWM];
+};
+
+template <typename T, int KWM>
+__device__ __forceinline__ void load_taps(const T *__restrict__ taps, int r, Taps<T, KWM> &t) {
+  const T *p = taps + (size_t)r * (3 * KWM);
+#pragma unroll
+  for (int i = 0; i < KWM; ++i) {
+    t.x[i] = p[i];
+    t.y[i] = p[KWM + i];
+    t.z[i] = p[2 * KWM + i];
+  }
+}
+
+// a[q] for a runtime q < KWM, by selects.  Each value passes through an empty
+// asm so the compiler cannot fold the select chain back into an indexed load
+// from a stack copy of the array (which would go to scratch memory).
+template <typename T, int KWM> __device__ __forceinline__ T sel_tap(const T *a, int q) {
+  T r = a[0];
+  __asm__ volatile("" : "+v"(r));
+#pragma unroll
+  for (int i = 1; i < KWM; ++i) {
+    T v = a[i];
+    __asm__ volatile("" : "+v"(v));
+    r = (q == i) ? v : r;
+  }
+  return r;
+}
+
 // Spread of one record into the workgroup's grid rows [row0, row0 + rpw) of plane p (LDS).
 template <typename T, int KWM>
-__device__ __forceinline__ void spread_rows(const Rec<T> &rc, T *ldsf, const Geom &g, int p, int row0,
-                                            const Ker<T> &K) {
+__device__ __forceinline__ void spread_rows(const Rec<T> rc, const Taps<T, KWM> &tp, T *ldsf, const Geom &g, int p,
+                                            int row0) {
   const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2, rpw = g.rpw;
   int k = p - rc.iw;
   if (k < 0) k += g.P;
@@ -198,15 +231,13 @@ __device__ __forceinline__ void spread_rows(const Rec<T> &rc, T *ldsf, const Geo
   int ry = rc.iy - row0;
   if (ry < 0) ry += nf2;
   if (!(ry < rpw || ry + kw - 1 >= nf2)) return;
-  const T vw = K.w(rc.dw, k);
+  const T vw = sel_tap<T, KWM>(tp.z, k);
   const T vre = rc.re * vw, vim = rc.im * vw;
-  T kx[KWM];
-  K.template taps<KWM>(rc.dx, kx);
   for (int jj = 0; jj < kw; ++jj) {
     int cy = ry + jj;
     if (cy >= nf2) cy -= nf2;
     if (cy >= rpw) continue;
-    const T ky = K(rc.dy, jj);
+    const T ky = sel_tap<T, KWM>(tp.y, jj);
     const T are = vre * ky, aim = vim * ky;
     const int base = cy * nf1;
 #pragma unroll
@@ -215,8 +246,8 @@ __device__ __forceinline__ void spread_rows(const Rec<T> &rc, T *ldsf, const Geo
         int cx = rc.ix + i;
         if (cx >= nf1) cx -= nf1;
         const int a = 2 * fftc::pad(base + cx);
-        atomicAdd(ldsf + a, are * kx[i]);
-        atomicAdd(ldsf + a + 1, aim * kx[i]);
+        atomicAdd(ldsf + a, are * tp.x[i]);
+        atomicAdd(ldsf + a + 1, aim * tp.x[i]);
       }
     }
   }
@@ -225,8 +256,8 @@ __device__ __forceinline__ void spread_rows(const Rec<T> &rc, T *ldsf, const Geo
 // Interpolation of one record from the workgroup's rows of plane p (LDS): its
 // partial sum over those rows, added to the record's FP64 accumulator.
 template <typename T, int KWM>
-__device__ __forceinline__ void interp_rows(const Rec<T> &rc, int idx, const cpx<T> *lds, const Geom &g, int p,
-                                            int row0, double2 *__restrict__ vacc, const Ker<T> &K) {
+__device__ __forceinline__ void interp_rows(const Rec<T> rc, const Taps<T, KWM> &tp, int idx, const cpx<T> *lds,
+                                            const Geom &g, int p, int row0, double2 *__restrict__ vacc) {
   const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2, rpw = g.rpw;
   int k = p - rc.iw;
   if (k < 0) k += g.P;
@@ -234,14 +265,11 @@ __device__ __forceinline__ void interp_rows(const Rec<T> &rc, int idx, const cpx
   int ry = rc.iy - row0;
   if (ry < 0) ry += nf2;
   if (!(ry < rpw || ry + kw - 1 >= nf2)) return;
-  T kx[KWM];
-  K.template taps<KWM>(rc.dx, kx);
   T sr = 0, si = 0;
   for (int jj = 0; jj < kw; ++jj) {
     int cy = ry + jj;
     if (cy >= nf2) cy -= nf2;
     if (cy >= rpw) continue;
-    const T ky = K(rc.dy, jj);
     const int base = cy * nf1;
     T rr = 0, ri = 0;
 #pragma unroll
@@ -250,24 +278,33 @@ __device__ __forceinline__ void interp_rows(const Rec<T> &rc, int idx, const cpx
         int cx = rc.ix + i;
         if (cx >= nf1) cx -= nf1;
         const cpx<T> f = lds[fftc::pad(base + cx)];
-        rr += f.x * kx[i];
-        ri += f.y * kx[i];
+        rr += f.x * tp.x[i];
+        ri += f.y * tp.x[i];
       }
     }
+    const T ky = sel_tap<T, KWM>(tp.y, jj);
     sr += rr * ky;
     si += ri * ky;
   }
-  const T kz = K.w(rc.dw, k);
+  const T kz = sel_tap<T, KWM>(tp.z, k);
   atomicAdd(&vacc[idx].x, (double)(sr * kz));
   atomicAdd(&vacc[idx].y, (double)(si * kz));
 }
 
 
 // The intermediate planes live in a column-tiled layout: element (plane k,
-// grid row k2, column slot s) at ((k * nct + s / kTile) * nf2 + k2) * kTile
-// + s % kTile, so a row's kTile-column chunks are contiguous for the row
-// kernels and a column's elements are kTile apart for the column kernels (the
-// kTile columns of a tile run on one XCD, whose L2 serves the shared sectors).
+// grid row k2, column slot s) at (k * nct + s / kTile) * tstride + k2 * kTile
+// + s % kTile with tstride = nf2 * kTile + a pad, so a row's kTile-column
+// chunks are contiguous for the row kernels and a column's elements are kTile
+// apart for the column kernels (the kTile columns of a tile run on one XCD,
+// whose L2 serves the shared sectors).
+
+// Offsets in H: the first element of plane k, and of column slot s in row 0
+// of a plane (row k2 adds k2 * kTile).
+__device__ __forceinline__ size_t h_plane(const Geom &g, int k) { return (size_t)k * g.nct * g.tstride; }
+__device__ __forceinline__ int h_slot(const Geom &g, int s) {
+  return (s >> kTileLog2) * g.tstride + (s & (kTile - 1));
+}
 
 // Column slots of the tiled layout.  With whole rows the slot of image column
 // col is col.  With rows split S ways, the columns of part h (grid index h mod
@@ -296,6 +333,15 @@ __device__ __forceinline__ int xcd_group(int bid, int n) {
   return x * q + (x < r ? x : r) + k;
 }
 
+// Workgroup id -> work item with blocks of blk consecutive items dealt to the
+// 8 XCDs in turn (n a multiple of 8 blk): every XCD gets the same mix of the
+// items, and a block's items share one L2.
+__device__ __forceinline__ int xcd_interleave(int bid, int n, int blk) {
+  if (n % (8 * blk)) return xcd_group(bid, n);
+  const int x = bid & 7, k = bid >> 3;
+  return ((k / blk) * 8 + x) * blk + k % blk;
+}
+
 template <typename T> struct MaxThreads { static constexpr int value = sizeof(T) == 8 ? 512 : 1024; };
 
 // ms2dirty: spread into full grid rows held in LDS, then the row FFT (length
@@ -304,6 +350,7 @@ template <typename T> struct MaxThreads { static constexpr int value = sizeof(T)
 // binned into full-width strips of T2 rows.
 template <typename T, int E, int KWM>
 __global__ void __launch_bounds__(MaxThreads<T>::value) spread_row_fft_kernel(const Rec<T> *__restrict__ recs,
+                                                                             const T *__restrict__ taps,
                                                                              const int *__restrict__ off,
                                                                              cpx<T> *__restrict__ H,
                                                                              const cpx<T> *__restrict__ tw, Geom g,
@@ -321,26 +368,23 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) spread_row_fft_kernel(co
   if (threadIdx.x < 64) walk_table_build(wt, off, g, p, row0, rpw);
   for (int i = threadIdx.x; i < ntot; i += nt) lds[i] = fftc::mk((T)0, (T)0);
   __syncthreads();
-  const Ker<T> K = make_ker<T>(g);
   const int total = wt.total;
   const int N1 = g.N1, nout = rpw * N1, h1 = N1 / 2;
-  const int lgt = 31 - __clz(g.nf2) + kTileLog2;  // log2(nf2 * kTile)
-  cpx<T> *hb = H + ((size_t)pl * g.nct * g.nf2 + row0) * kTile;
+  cpx<T> *hb = H + h_plane(g, pl) + row0 * kTile;
   if (total == 0) {
     // no visibility reaches these rows of this plane: the transform is zero
     for (int i = threadIdx.x; i < nout; i += nt) {
       const int r = rpw == 1 ? 0 : i / N1, col = i - r * N1;
-      hb[((col >> kTileLog2) << lgt) + (r << kTileLog2) + (col & (kTile - 1))] = fftc::mk((T)0, (T)0);
+      hb[h_slot(g, col) + (r << kTileLog2)] = fftc::mk((T)0, (T)0);
     }
     return;
   }
-  for (int t0 = threadIdx.x; t0 < total; t0 += 2 * nt) {
-    const int t1 = t0 + nt;
-    const Rec<T> ra = recs[walk_record(wt, t0)];
-    Rec<T> rb;
-    if (t1 < total) rb = recs[walk_record(wt, t1)];
-    spread_rows<T, KWM>(ra, ldsf, g, p, row0, K);
-    if (t1 < total) spread_rows<T, KWM>(rb, ldsf, g, p, row0, K);
+  for (int t0 = threadIdx.x; t0 < total; t0 += nt) {
+    const int ia = walk_record(wt, t0);
+    const Rec<T> ra = recs[ia];
+    Taps<T, KWM> ta;
+    load_taps<T, KWM>(taps, ia, ta);
+    spread_rows<T, KWM>(ra, ta, ldsf, g, p, row0);
   }
   __syncthreads();
   lds_fft_rows<T, E, 16>(lds, 31 - __clz(nf1), rpw, +1, tv);
@@ -363,7 +407,7 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) spread_row_fft_kernel(co
     const int i = tid + e * nt;
     if (i < nout) {
       const int r = rpw == 1 ? 0 : i / N1, col = i - r * N1;
-      hb[((col >> kTileLog2) << lgt) + (r << kTileLog2) + (col & (kTile - 1))] = o[e];
+      hb[h_slot(g, col) + (r << kTileLog2)] = o[e];
     }
   }
 }
@@ -376,6 +420,7 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) spread_row_fft_kernel(co
 template <typename T, int E, int KWM>
 __global__ void __launch_bounds__(MaxThreads<T>::value) row_ifft_interp_kernel(const cpx<T> *__restrict__ H,
                                                                               const Rec<T> *__restrict__ recs,
+                                                                              const T *__restrict__ taps,
                                                                               const int *__restrict__ off,
                                                                               double2 *__restrict__ vacc,
                                                                               const cpx<T> *__restrict__ tw,
@@ -386,16 +431,15 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) row_ifft_interp_kernel(c
   const int rpw = g.rpw, nf1 = g.nf1, N1 = g.N1, nt = blockDim.x;
   // rows in XCD-contiguous order: the 16 rows sharing a column line run together
   const int row0 = xcd_group(blockIdx.x, gridDim.x) * rpw, pl = blockIdx.y, p = p0 + pl;
-  const int lg = 31 - __clz(nf1), lg2 = 31 - __clz(g.nf2);
+  const int lg = 31 - __clz(nf1);
   cpx<T> v[E];
-  const cpx<T> *hb = H + ((size_t)pl * g.nct * g.nf2 + row0) * kTile;
-  const int lgt = lg2 + kTileLog2;
+  const cpx<T> *hb = H + h_plane(g, pl) + row0 * kTile;
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int i = threadIdx.x + e * nt;  // < rpw * nf1 = E * nt
     const int r = i >> lg, k1 = i & (nf1 - 1);
     const int col = image_index(k1, N1, nf1);
-    v[e] = col >= 0 ? hb[((col >> kTileLog2) << lgt) + (r << kTileLog2) + (col & (kTile - 1))]
+    v[e] = col >= 0 ? hb[h_slot(g, col) + (r << kTileLog2)]
                     : fftc::mk((T)0, (T)0);
   }
   const auto tv = lds_twiddles<T>(lds + fftc::padded_size(rpw * nf1), tw, nf1);
@@ -408,16 +452,19 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) row_ifft_interp_kernel(c
   // first record of this thread, loaded while the FFT runs
   int idx0 = 0;
   Rec<T> r0;
+  Taps<T, KWM> q0;
   if (threadIdx.x < total) {
     idx0 = walk_record(wt, threadIdx.x);
     r0 = recs[idx0];
+    load_taps<T, KWM>(taps, idx0, q0);
   }
   lds_fft_rows<T, E, 16>(lds, lg, rpw, sign, tv);
-  const Ker<T> K = make_ker<T>(g);
-  if (threadIdx.x < total) interp_rows<T, KWM>(r0, idx0, lds, g, p, row0, vacc, K);
+  if (threadIdx.x < total) interp_rows<T, KWM>(r0, q0, idx0, lds, g, p, row0, vacc);
   for (int t = threadIdx.x + nt; t < total; t += nt) {
     const int idx = walk_record(wt, t);
-    interp_rows<T, KWM>(recs[idx], idx, lds, g, p, row0, vacc, K);
+    Taps<T, KWM> q;
+    load_taps<T, KWM>(taps, idx, q);
+    interp_rows<T, KWM>(recs[idx], q, idx, lds, g, p, row0, vacc);
   }
 }
 
@@ -425,9 +472,10 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) row_ifft_interp_kernel(c
 // holds z_h[n mod m] = sum over the row's cells n of x[n] w^(sign h n), whose
 // m-point FFT gives the row transform at the indices S k + h (radix-S
 // decimation in frequency; w = e^{2 pi i / nf1}, m = nf1 / S).
-template <typename T, int KWM, typename Tw>
-__device__ __forceinline__ void spread_row_part(const Rec<T> &rc, T *ldsf, const Tw &tv, const Geom &g, int p,
-                                                int row0, int h, int mm, int sign, const Ker<T> &K) {
+template <typename T, int KWM, bool PAD = true, typename Tw>
+__device__ __forceinline__ void spread_row_part(const Rec<T> rc, const Taps<T, KWM> &tp, T *ldsf, const Tw &tv,
+                                                const cpx<T> *wh, const Geom &g, int p, int row0, int h, int mm,
+                                                int sign) {
   const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2;
   int k = p - rc.iw;
   if (k < 0) k += g.P;
@@ -435,24 +483,27 @@ __device__ __forceinline__ void spread_row_part(const Rec<T> &rc, T *ldsf, const
   int jj = row0 - rc.iy;
   if (jj < 0) jj += nf2;
   if (jj >= kw) return;
-  const T a = K.w(rc.dw, k) * K(rc.dy, jj);
-  const T vre = rc.re * a, vim = rc.im * a;
-  T kxs[KWM];
-  K.template taps<KWM>(rc.dx, kxs);
+  const T a = sel_tap<T, KWM>(tp.z, k) * sel_tap<T, KWM>(tp.y, jj);
+  T vre = rc.re * a, vim = rc.im * a;
+  if (h) {
+    // w^(sign h n) at the cells n = ix + i: w^(sign h ix) once, w^(sign h i) per tap (wh)
+    const cpx<T> w = tv[(h * rc.ix) & (nf1 - 1)];
+    const T ws = (T)sign * w.y, t = vre * w.x - vim * ws;
+    vim = vre * ws + vim * w.x;
+    vre = t;
+  }
 #pragma unroll
   for (int i = 0; i < KWM; ++i) {
     if (i < kw) {
       int cx = rc.ix + i;
       if (cx >= nf1) cx -= nf1;
-      const T kx = kxs[i];
-      T re = vre * kx, im = vim * kx;
+      T re = vre * tp.x[i], im = vim * tp.x[i];
       if (h) {
-        const cpx<T> w = tv[(h * cx) & (nf1 - 1)];
-        const T ws = (T)sign * w.y, t = re * w.x - im * ws;
-        im = re * ws + im * w.x;
+        const T t = re * wh[i].x - im * wh[i].y;
+        im = re * wh[i].y + im * wh[i].x;
         re = t;
       }
-      const int q = 2 * fftc::pad(cx & mm);
+      const int q = 2 * (PAD ? fftc::pad(cx & mm) : (cx & mm));
       atomicAdd(ldsf + q, re);
       atomicAdd(ldsf + q + 1, im);
     }
@@ -467,15 +518,16 @@ __device__ __forceinline__ void spread_row_part(const Rec<T> &rc, T *ldsf, const
 // memory phases overlap another's FFT.
 template <typename T, int E, int KWM, int S>
 __global__ void __launch_bounds__(MaxThreads<T>::value / S) spread_row_fft_part_kernel(
-    const Rec<T> *__restrict__ recs, const int *__restrict__ off, cpx<T> *__restrict__ H,
+    const Rec<T> *__restrict__ recs, const T *__restrict__ taps, const int *__restrict__ off, cpx<T> *__restrict__ H,
     const cpx<T> *__restrict__ tw, Geom g, int p0) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ WalkTable wt;
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
   T *ldsf = reinterpret_cast<T *>(smem_raw);
   const int nf1 = g.nf1, m = nf1 / S, nt = blockDim.x, sign = +1;
-  // the parts of a row are consecutive items: same XCD, adjacent in time
-  const int item = xcd_group(blockIdx.x, gridDim.x);
+  // the parts of four consecutive rows (whose chunks share lines of H) are
+  // consecutive items of one XCD; the XCDs take such blocks in turn
+  const int item = xcd_interleave(blockIdx.x, gridDim.x, 4 * S);
   const int row0 = item / S, h = item % S, pl = blockIdx.y, p = p0 + pl;
   const int ntot = fftc::padded_size(m);
   const auto tv = lds_twiddles<T>(lds + ntot, tw, nf1);
@@ -485,24 +537,28 @@ __global__ void __launch_bounds__(MaxThreads<T>::value / S) spread_row_fft_part_
   const int total = wt.total;
   // this part's image columns: grid_index(col) = col - N1/2 (mod nf1) = h (mod S)
   const int N1 = g.N1, h1 = N1 / 2, c0 = (h + h1) % S, nout = (N1 - c0 + S - 1) / S;
-  const int lgt = 31 - __clz(g.nf2) + kTileLog2;
-  cpx<T> *hb = H + ((size_t)pl * g.nct * g.nf2 + row0) * kTile;
+  cpx<T> *hb = H + h_plane(g, pl) + row0 * kTile;
   const int hs = h << g.slot_lg;  // this part's first column slot
   if (total == 0) {
     for (int i = threadIdx.x; i < nout; i += nt) {
       const int sl = hs + i;
-      hb[((sl >> kTileLog2) << lgt) + (sl & (kTile - 1))] = fftc::mk((T)0, (T)0);
+      hb[h_slot(g, sl)] = fftc::mk((T)0, (T)0);
     }
     return;
   }
-  const Ker<T> K = make_ker<T>(g);
-  for (int t0 = threadIdx.x; t0 < total; t0 += 2 * nt) {
-    const int t1 = t0 + nt;
-    const Rec<T> ra = recs[walk_record(wt, t0)];
-    Rec<T> rb;
-    if (t1 < total) rb = recs[walk_record(wt, t1)];
-    spread_row_part<T, KWM>(ra, ldsf, tv, g, p, row0, h, m - 1, sign, K);
-    if (t1 < total) spread_row_part<T, KWM>(rb, ldsf, tv, g, p, row0, h, m - 1, sign, K);
+  // w^(sign h i) for the taps i (part h's decimation factor), from the LDS twiddles
+  cpx<T> wh[KWM];
+#pragma unroll
+  for (int i = 0; i < KWM; ++i) {
+    const cpx<T> w = tv[h * i];
+    wh[i] = fftc::mk(w.x, (T)sign * w.y);
+  }
+  for (int t0 = threadIdx.x; t0 < total; t0 += nt) {
+    const int ia = walk_record(wt, t0);
+    const Rec<T> ra = recs[ia];
+    Taps<T, KWM> ta;
+    load_taps<T, KWM>(taps, ia, ta);
+    spread_row_part<T, KWM>(ra, ta, ldsf, tv, wh, g, p, row0, h, m - 1, sign);
   }
   __syncthreads();
   lds_fft_part<T, E, 16, S>(lds, 31 - __clz(m), sign, tv);
@@ -524,7 +580,7 @@ __global__ void __launch_bounds__(MaxThreads<T>::value / S) spread_row_fft_part_
     const int i = tid + e * nt;
     if (i < nout) {
       const int sl = hs + i;
-      hb[((sl >> kTileLog2) << lgt) + (sl & (kTile - 1))] = o[e];
+      hb[h_slot(g, sl)] = o[e];
     }
   }
 }
@@ -533,10 +589,11 @@ __global__ void __launch_bounds__(MaxThreads<T>::value / S) spread_row_fft_part_
 // cells are x[n] = sum_h w^(sign h n) Y_h[n mod m] (w = e^{2 pi i / nf1}, m = nf1 / S),
 // and part h holds Y_h; its share of the record's sum is added to the record's
 // FP64 accumulator.
-template <typename T, int KWM, typename Tw>
-__device__ __forceinline__ void interp_row_part(const Rec<T> &rc, int idx, const cpx<T> *lds, const Tw &tv,
-                                                const Geom &g, int p, int row0, int h, int mm, int sign,
-                                                double2 *__restrict__ vacc, const Ker<T> &K) {
+template <typename T, int KWM, bool PAD = true, typename Tw>
+__device__ __forceinline__ void interp_row_part(const Rec<T> rc, const Taps<T, KWM> &tp, int idx,
+                                                const cpx<T> *lds, const Tw &tv, const cpx<T> *wh, const Geom &g,
+                                                int p, int row0, int h, int mm, int sign,
+                                                double2 *__restrict__ vacc) {
   const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2;
   int k = p - rc.iw;
   if (k < 0) k += g.P;
@@ -545,24 +602,24 @@ __device__ __forceinline__ void interp_row_part(const Rec<T> &rc, int idx, const
   if (jj < 0) jj += nf2;
   if (jj >= kw) return;
   T rr = 0, ri = 0;
-  T kxs[KWM];
-  K.template taps<KWM>(rc.dx, kxs);
 #pragma unroll
   for (int i = 0; i < KWM; ++i) {
     if (i < kw) {
       int cx = rc.ix + i;
       if (cx >= nf1) cx -= nf1;
-      const T kx = kxs[i];
-      cpx<T> f = lds[fftc::pad(cx & mm)];
-      if (h) {
-        const cpx<T> w = tv[(h * cx) & (nf1 - 1)];
-        f = fftc::mulw(f, w.x, (T)sign * w.y);
-      }
-      rr += f.x * kx;
-      ri += f.y * kx;
+      cpx<T> f = lds[PAD ? fftc::pad(cx & mm) : (cx & mm)];
+      if (h) f = fftc::mulw(f, wh[i].x, wh[i].y);  // w^(sign h i); w^(sign h ix) below
+      rr += f.x * tp.x[i];
+      ri += f.y * tp.x[i];
     }
   }
-  const T kyz = K(rc.dy, jj) * K.w(rc.dw, k);
+  if (h) {
+    const cpx<T> w = tv[(h * rc.ix) & (nf1 - 1)];
+    const T ws = (T)sign * w.y, t = rr * w.x - ri * ws;
+    ri = rr * ws + ri * w.x;
+    rr = t;
+  }
+  const T kyz = sel_tap<T, KWM>(tp.y, jj) * sel_tap<T, KWM>(tp.z, k);
   atomicAdd(&vacc[idx].x, (double)(rr * kyz));
   atomicAdd(&vacc[idx].y, (double)(ri * kyz));
 }
@@ -574,7 +631,8 @@ __device__ __forceinline__ void interp_row_part(const Rec<T> &rc, int idx, const
 // FFTs and add their shares to each record's accumulator.
 template <typename T, int E, int KWM, int S>
 __global__ void __launch_bounds__(MaxThreads<T>::value / S) row_ifft_interp_part_kernel(
-    const cpx<T> *__restrict__ H, const Rec<T> *__restrict__ recs, const int *__restrict__ off,
+    const cpx<T> *__restrict__ H, const Rec<T> *__restrict__ recs, const T *__restrict__ taps,
+    const int *__restrict__ off,
     double2 *__restrict__ vacc, const cpx<T> *__restrict__ tw, Geom g, int p0, int sign) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ WalkTable wt;
@@ -582,14 +640,14 @@ __global__ void __launch_bounds__(MaxThreads<T>::value / S) row_ifft_interp_part
   const int nf1 = g.nf1, m = nf1 / S, N1 = g.N1, nt = blockDim.x;
   const int item = xcd_group(blockIdx.x, gridDim.x);
   const int row0 = item / S, h = item % S, pl = blockIdx.y, p = p0 + pl;
-  const int lgm = 31 - __clz(m), lgt = 31 - __clz(g.nf2) + kTileLog2;
-  const cpx<T> *hb = H + ((size_t)pl * g.nct * g.nf2 + row0) * kTile;
+  const int lgm = 31 - __clz(m);
+  const cpx<T> *hb = H + h_plane(g, pl) + row0 * kTile;
   cpx<T> v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int i = threadIdx.x + e * nt;  // < m = E * nt
     const int col = image_index(S * i + h, N1, nf1), sl = (h << g.slot_lg) + col / S;
-    v[e] = col >= 0 ? hb[((sl >> kTileLog2) << lgt) + (sl & (kTile - 1))] : fftc::mk((T)0, (T)0);
+    v[e] = col >= 0 ? hb[h_slot(g, sl)] : fftc::mk((T)0, (T)0);
   }
   const auto tv = lds_twiddles<T>(lds + fftc::padded_size(m), tw, nf1);
   if (threadIdx.x < 64) walk_table_build(wt, off, g, p, row0, 1);
@@ -600,17 +658,25 @@ __global__ void __launch_bounds__(MaxThreads<T>::value / S) row_ifft_interp_part
   if (total == 0) return;
   int idx0 = 0;
   Rec<T> r0;
+  Taps<T, KWM> q0;
   if (threadIdx.x < total) {
     idx0 = walk_record(wt, threadIdx.x);
     r0 = recs[idx0];
+    load_taps<T, KWM>(taps, idx0, q0);
   }
   lds_fft_part<T, E, 16, S>(lds, lgm, sign, tv);
-  const Ker<T> K = make_ker<T>(g);
-  if (threadIdx.x < total)
-    interp_row_part<T, KWM>(r0, idx0, lds, tv, g, p, row0, h, m - 1, sign, vacc, K);
+  cpx<T> wh[KWM];
+#pragma unroll
+  for (int i = 0; i < KWM; ++i) {
+    const cpx<T> w = tv[h * i];
+    wh[i] = fftc::mk(w.x, (T)sign * w.y);
+  }
+  if (threadIdx.x < total) interp_row_part<T, KWM>(r0, q0, idx0, lds, tv, wh, g, p, row0, h, m - 1, sign, vacc);
   for (int t = threadIdx.x + nt; t < total; t += nt) {
     const int idx = walk_record(wt, t);
-    interp_row_part<T, KWM>(recs[idx], idx, lds, tv, g, p, row0, h, m - 1, sign, vacc, K);
+    Taps<T, KWM> q;
+    load_taps<T, KWM>(taps, idx, q);
+    interp_row_part<T, KWM>(recs[idx], q, idx, lds, tv, wh, g, p, row0, h, m - 1, sign, vacc);
   }
 }
 
@@ -638,7 +704,7 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) col_fwd_kernel(const cpx
   const int cpw = g.cpw, nf2 = g.nf2, N2 = g.N2, nt = blockDim.x;
   const int s0 = xcd_group(blockIdx.x, gridDim.x) * cpw;  // first column slot
   if (!slots_used(s0, g)) return;
-  const int lg = 31 - __clz(nf2), lgt = lg + kTileLog2;
+  const int lg = 31 - __clz(nf2);
   const int nout = cpw * N2;
   double ar[8], ai[8], zp[8];
 #pragma unroll
@@ -656,13 +722,13 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) col_fwd_kernel(const cpx
     // the loop rather than held in registers across it
     int tid = threadIdx.x;
     __asm__ volatile("" : "+v"(tid));
-    const cpx<T> *hk = H + (size_t)k * g.nct * nf2 * kTile;
+    const cpx<T> *hk = H + h_plane(g, k);
     cpx<T> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = tid + e * nt;
       const int ci = i >> lg, k2 = i & (nf2 - 1), sl = s0 + ci;
-      v[e] = slot_column(sl, g) >= 0 ? hk[((sl >> kTileLog2) << lgt) + (k2 << kTileLog2) + (sl & (kTile - 1))]
+      v[e] = slot_column(sl, g) >= 0 ? hk[h_slot(g, sl) + (k2 << kTileLog2)]
                                      : fftc::mk((T)0, (T)0);
     }
     __syncthreads();  // the previous plane's LDS reads are done
@@ -770,12 +836,12 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) col_inv_kernel(const cpx
     cpx<T> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = lds[fftc::pad(tid2 + e * nt)];
-    cpx<T> *hk = H + (size_t)k * g.nct * nf2 * kTile;
+    cpx<T> *hk = H + h_plane(g, k);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = tid2 + e * nt;
       const int ci = i >> lg, k2 = i & (nf2 - 1), sl = s0 + ci;
-      if (slot_column(sl, g) >= 0) hk[((sl >> kTileLog2) << (lg + kTileLog2)) + (k2 << kTileLog2) + (sl & (kTile - 1))] = v[e];
+      if (slot_column(sl, g) >= 0) hk[h_slot(g, sl) + (k2 << kTileLog2)] = v[e];
     }
   }
 }
@@ -794,23 +860,23 @@ inline int col_groups(const Geom &g) { return (g.nslot + g.cpw - 1) / g.cpw; }
 
 // ms2dirty, one batch on the LDS path: spread + row FFT -> H (column-tiled)
 template <typename T>
-void lds_spread_rows(const Rec<T> *recs, const int *off, const Geom &g, int p0, int nb, cpx<T> *H,
+void lds_spread_rows(const Rec<T> *recs, const T *taps, const int *off, const Geom &g, int p0, int nb, cpx<T> *H,
                      const cpx<T> *tw1, hipStream_t st) {
   if (g.rsplit == kRowSplit) {
     const dim3 grd(kRowSplit * g.nf2, nb);
     const int nt = g.nf1 / (16 * kRowSplit), lb = g.lds_rowp_bytes;
     if (g.kw <= 8)
-      spread_row_fft_part_kernel<T, 16, 8, kRowSplit><<<grd, nt, lb, st>>>(recs, off, H, tw1, g, p0);
+      spread_row_fft_part_kernel<T, 16, 8, kRowSplit><<<grd, nt, lb, st>>>(recs, taps, off, H, tw1, g, p0);
     else
-      spread_row_fft_part_kernel<T, 16, 16, kRowSplit><<<grd, nt, lb, st>>>(recs, off, H, tw1, g, p0);
+      spread_row_fft_part_kernel<T, 16, 16, kRowSplit><<<grd, nt, lb, st>>>(recs, taps, off, H, tw1, g, p0);
     return;
   }
   const dim3 grd(g.nf2 / g.rpw, nb);
   const int nt = g.rpw * g.nf1 / 16;
   if (g.kw <= 8)
-    spread_row_fft_kernel<T, 16, 8><<<grd, nt, g.lds_row_bytes, st>>>(recs, off, H, tw1, g, p0);
+    spread_row_fft_kernel<T, 16, 8><<<grd, nt, g.lds_row_bytes, st>>>(recs, taps, off, H, tw1, g, p0);
   else
-    spread_row_fft_kernel<T, 16, 16><<<grd, nt, g.lds_row_bytes, st>>>(recs, off, H, tw1, g, p0);
+    spread_row_fft_kernel<T, 16, 16><<<grd, nt, g.lds_row_bytes, st>>>(recs, taps, off, H, tw1, g, p0);
 }
 
 template <typename T>
@@ -826,23 +892,23 @@ int allow_spread_rows(const Geom &g) {
 
 // dirty2ms, one batch on the LDS path: inverse row FFT fused with the interpolation
 template <typename T>
-void lds_row_interp(const Rec<T> *recs, const int *off, const cpx<T> *H, double2 *vacc, const cpx<T> *tw1,
-                    const Geom &g, int p0, int nb, hipStream_t st) {
+void lds_row_interp(const Rec<T> *recs, const T *taps, const int *off, const cpx<T> *H, double2 *vacc,
+                    const cpx<T> *tw1, const Geom &g, int p0, int nb, hipStream_t st) {
   if (g.rsplit == kRowSplit) {
     const dim3 grd(kRowSplit * g.nf2, nb);
     const int nt = g.nf1 / (16 * kRowSplit), lb = g.lds_rowp_bytes;
     if (g.kw <= 8)
-      row_ifft_interp_part_kernel<T, 16, 8, kRowSplit><<<grd, nt, lb, st>>>(H, recs, off, vacc, tw1, g, p0, -1);
+      row_ifft_interp_part_kernel<T, 16, 8, kRowSplit><<<grd, nt, lb, st>>>(H, recs, taps, off, vacc, tw1, g, p0, -1);
     else
-      row_ifft_interp_part_kernel<T, 16, 16, kRowSplit><<<grd, nt, lb, st>>>(H, recs, off, vacc, tw1, g, p0, -1);
+      row_ifft_interp_part_kernel<T, 16, 16, kRowSplit><<<grd, nt, lb, st>>>(H, recs, taps, off, vacc, tw1, g, p0, -1);
     return;
   }
   const dim3 grd(g.nf2 / g.rpw, nb);
   const int nt = g.rpw * g.nf1 / 16;
   if (g.kw <= 8)
-    row_ifft_interp_kernel<T, 16, 8><<<grd, nt, g.lds_row_bytes, st>>>(H, recs, off, vacc, tw1, g, p0, -1);
+    row_ifft_interp_kernel<T, 16, 8><<<grd, nt, g.lds_row_bytes, st>>>(H, recs, taps, off, vacc, tw1, g, p0, -1);
   else
-    row_ifft_interp_kernel<T, 16, 16><<<grd, nt, g.lds_row_bytes, st>>>(H, recs, off, vacc, tw1, g, p0, -1);
+    row_ifft_interp_kernel<T, 16, 16><<<grd, nt, g.lds_row_bytes, st>>>(H, recs, taps, off, vacc, tw1, g, p0, -1);
 }
 
 template <typename T>
@@ -879,12 +945,12 @@ int allow_cols(const Geom &g) {
 
 // one source per (precision, kernel group), so the heavy kernels compile in parallel
 #define CURIG_LDS_SPREAD_INST(T)                                                                                   \
-  template void lds_spread_rows<T>(const Rec<T> *, const int *, const Geom &, int, int, cpx<T> *, const cpx<T> *,  \
-                                   hipStream_t);                                                                   \
+  template void lds_spread_rows<T>(const Rec<T> *, const T *, const int *, const Geom &, int, int, cpx<T> *,       \
+                                   const cpx<T> *, hipStream_t);                                                   \
   template int allow_spread_rows<T>(const Geom &);
 #define CURIG_LDS_INTERP_INST(T)                                                                                   \
-  template void lds_row_interp<T>(const Rec<T> *, const int *, const cpx<T> *, double2 *, const cpx<T> *,          \
-                                  const Geom &, int, int, hipStream_t);                                            \
+  template void lds_row_interp<T>(const Rec<T> *, const T *, const int *, const cpx<T> *, double2 *,              \
+                                  const cpx<T> *, const Geom &, int, int, hipStream_t);                            \
   template int allow_row_interp<T>(const Geom &);
 #define CURIG_LDS_COLS_INST(T)                                                                                     \
   template void lds_cols_forward<T>(const Geom &, int, int, const cpx<T> *, double2 *, const cpx<T> *, int,        \

@@ -158,13 +158,18 @@ __global__ void __launch_bounds__(kScanBlock) scan_apply_kernel(const int *__res
 
 // Scatter rows into bin order; the value carries the weight and e^{+i D w}
 // (ms2dirty pre-stages, src/astro/ra_exec.cu:98-110).
-template <typename T>
+// taps (LDS path, else null): each record's kernel values at its KP cells per
+// axis, [nrec][3][KP] (x, y, w; zero beyond the kernel width; the w taps are
+// 1, 0, ... in the 2-D form), evaluated from the rounded offsets stored in the
+// record, so the row kernels read what they would compute.
+template <typename T, int KP>
 __global__ void __launch_bounds__(256) bin_scatter_kernel(const double *__restrict__ uvw, int nrow,
                                                            const void *__restrict__ vis, long vs,
                                                            const void *__restrict__ wgt, long ws, Geom g,
                                                            const int2 *__restrict__ binrank,
                                                            const int *__restrict__ off, Rec<T> *__restrict__ recs,
-                                                           int *__restrict__ perm, int with_value) {
+                                                           int *__restrict__ perm, int with_value,
+                                                           T *__restrict__ taps) {
   for (long j = blockIdx.x * (long)blockDim.x + threadIdx.x; j < nrow; j += (long)gridDim.x * blockDim.x) {
     int2 br = binrank[j];
     if (br.x < 0) continue;
@@ -190,6 +195,25 @@ __global__ void __launch_bounds__(256) bin_scatter_kernel(const double *__restri
     }
     recs[dst] = r;
     perm[dst] = (int)j;
+    if (taps) {
+      const Ker<T> K = make_ker<T>(g);
+      T kx[KP], ky[KP], kz[KP];
+      K.template taps<KP>(r.dx, kx);
+      K.template taps<KP>(r.dy, ky);
+      if (g.kww == 1) {
+#pragma unroll
+        for (int i = 0; i < KP; ++i) kz[i] = i == 0 ? (T)1 : (T)0;
+      } else {
+        K.template taps<KP>(r.dw, kz);
+      }
+      T *o = taps + (size_t)dst * (3 * KP);
+#pragma unroll
+      for (int i = 0; i < KP; ++i) {
+        o[i] = kx[i];
+        o[KP + i] = ky[i];
+        o[2 * KP + i] = kz[i];
+      }
+    }
   }
 }
 
@@ -226,10 +250,18 @@ void enqueue_binning(const Problem &pb, const ChanPlan &cp, const Bufs &b, hipSt
   scan_reduce_kernel<<<(int)nscan, kScanBlock, 0, st>>>(b.hist, nbins, b.bsum);
   scan_sums_kernel<<<1, 1024, 0, st>>>(b.bsum, (int)nscan);
   scan_apply_kernel<<<(int)nscan, kScanBlock, 0, st>>>(b.hist, nbins, b.bsum, b.off, (int)cp.nvalid);
-  if (pb.nrow > 0)
-    bin_scatter_kernel<T><<<nblocks(pb.nrow), 256, 0, st>>>(pb.uvw, pb.nrow, pb.vis, pb.vis_stride, pb.wgt,
-                                                             pb.wgt_stride, g, b.binrank, b.off, (Rec<T> *)b.recs,
-                                                             b.perm, with_value ? 1 : 0);
+  if (pb.nrow > 0) {
+    T *taps = g.fft_lds ? (T *)b.taps : nullptr;
+    if (tap_width(g.kw) == 8)
+      bin_scatter_kernel<T, 8><<<nblocks(pb.nrow), 256, 0, st>>>(pb.uvw, pb.nrow, pb.vis, pb.vis_stride, pb.wgt,
+                                                                  pb.wgt_stride, g, b.binrank, b.off,
+                                                                  (Rec<T> *)b.recs, b.perm, with_value ? 1 : 0, taps);
+    else
+      bin_scatter_kernel<T, 16><<<nblocks(pb.nrow), 256, 0, st>>>(pb.uvw, pb.nrow, pb.vis, pb.vis_stride, pb.wgt,
+                                                                   pb.wgt_stride, g, b.binrank, b.off,
+                                                                   (Rec<T> *)b.recs, b.perm, with_value ? 1 : 0,
+                                                                   taps);
+  }
 }
 
 template void enqueue_binning<float>(const Problem &, const ChanPlan &, const Bufs &, hipStream_t, bool);

@@ -48,11 +48,16 @@ struct Bufs {
   int *hist = nullptr, *off = nullptr, *bsum = nullptr, *perm = nullptr;
   int2 *binrank = nullptr;
   void *recs = nullptr, *grid = nullptr, *H = nullptr;
+  void *taps = nullptr;  // LDS path: each record's kernel taps, [nrec][3][tap_width] of the precision
   double2 *acc = nullptr, *vacc = nullptr;
   const void *tw1 = nullptr, *tw2 = nullptr;
   const double *hc_d = nullptr;  // piecewise-polynomial kernel coefficients, FP64 | FP32
   const float *hc_f = nullptr;
 };
+
+// Taps stored per record and axis on the LDS path: the kernel width rounded up
+// to the row kernels' unrolled width (8 or 16).
+inline int tap_width(int kw) { return kw <= 8 ? 8 : 16; }
 
 inline int nblocks(long n, int bs = 256, long cap = 1 << 20) {
   long b = (n + bs - 1) / bs;
@@ -130,8 +135,8 @@ void launch_explicit_dirty2ms(bool dbl, const Problem &pb, const Geom &g, double
 // ---- kern_lds_*.hip: pruned LDS FFT path (kernel width <= 16) ---------------
 // ms2dirty, one batch: spread + row FFT -> H (column-tiled)
 template <typename T>
-void lds_spread_rows(const Rec<T> *recs, const int *off, const Geom &g, int p0, int nb, cpx<T> *H, const cpx<T> *tw1,
-                     hipStream_t st);
+void lds_spread_rows(const Rec<T> *recs, const T *taps, const int *off, const Geom &g, int p0, int nb, cpx<T> *H,
+                     const cpx<T> *tw1, hipStream_t st);
 // ms2dirty, one batch: column FFT + w-accumulation into accT
 template <typename T>
 void lds_cols_forward(const Geom &g, int p0, int nb, const cpx<T> *H, double2 *accT, const cpx<T> *tw2, int first,
@@ -141,8 +146,8 @@ template <typename T>
 void lds_cols_backward(const Geom &g, int p0, int nb, const cpx<T> *gT, cpx<T> *H, const cpx<T> *tw2, hipStream_t st);
 // dirty2ms, one batch: inverse row FFT fused with the interpolation
 template <typename T>
-void lds_row_interp(const Rec<T> *recs, const int *off, const cpx<T> *H, double2 *vacc, const cpx<T> *tw1,
-                    const Geom &g, int p0, int nb, hipStream_t st);
+void lds_row_interp(const Rec<T> *recs, const T *taps, const int *off, const cpx<T> *H, double2 *vacc,
+                    const cpx<T> *tw1, const Geom &g, int p0, int nb, hipStream_t st);
 // raise the dynamic-LDS limit of the LDS-path kernels a plan uses
 template <typename T> int allow_spread_rows(const Geom &g);
 template <typename T> int allow_row_interp(const Geom &g);

@@ -445,6 +445,67 @@ def test_split_rows_equal_rocfft_path(precision, n):
     assert O.l2error(want, np.asarray(v_lds)[sel]) < 3 * eps + (3e-6 if precision == "fp32" else 0)
 
 
+def test_split_rows_dense_and_empty_rows():
+    """FP32 rows of 16384 (8192^2 images) split in four parts, with visibilities
+    that leave whole bands of rows empty (the zero-output shortcut), put
+    thousands of records into a few rows (several per thread of a row-part
+    workgroup) and spread the rest over the grid: both directions equal the
+    rocFFT path and the FP64 direct sums."""
+    lib = _lib()
+    torch = _torch()
+    n, f0, eps = 8192, 1e9, 1e-6
+    say(f"[split rows, dense and empty rows] fp32 {n}x{n}")
+    rng = np.random.default_rng(17)
+    dx = 1.0 * np.pi / 180 / n
+    scale = C / f0 / dx
+    spread_rows = (rng.random((15000, 2)) - 0.5) * 0.98
+    dense = np.stack([(rng.random(15000) - 0.5) * 0.98, 0.2 + rng.random(15000) * 0.002], axis=1)
+    uv = np.concatenate([spread_rows, dense])
+    uv = uv[np.abs(uv[:, 0] - 0.1) > 0.05]  # an empty band of grid rows
+    nrow = len(uv)
+    uvw = np.empty((nrow, 3))
+    uvw[:, :2] = uv * scale
+    uvw[:, 2] = rng.uniform(-1, 1, nrow) * 30.0
+    wgt = rng.uniform(0.5, 1.0, nrow)
+    vis = rng.standard_normal(nrow) + 1j * rng.standard_normal(nrow)
+    img = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))).astype(np.complex64)
+    info = _lib_plan(lib, nrow, n, n, 1.0, f0, eps, uvw, "fp32")
+    assert info["fft_lds"] == 1 and info["rsplit"] == 4
+
+    def both(fft_path):
+        assert lib.curig_set_fft_path(fft_path) == 0
+        try:
+            return (gpu_f32(True, uvw, vis, wgt, n, n, 1.0, f0, eps),
+                    gpu_f32(False, uvw, img, wgt, n, n, 1.0, f0, eps))
+        finally:
+            lib.curig_set_fft_path(0)
+
+    d_lds, v_lds = both(0)
+    d_roc, v_roc = both(1)
+    e1 = O.l2error(d_roc, d_lds)
+    pix = rng.choice(n * n, 64, replace=False)
+    want = O.dft_ms2dirty_pixels(uvw, vis, wgt, n, n, 1.0, f0, pix, -1)
+    e2 = O.l2error(want, np.asarray(d_lds).reshape(-1)[pix])
+    say(f"  ms2dirty: vs rocFFT path {e1:.2e}, vs direct sum {e2:.2e}")
+    # dirty2ms against the GPU FP64 direct sum (pinned to the CPU oracle in
+    # test_gpu_configs.py)
+    e3 = O.l2error(v_roc, v_lds)
+    rows = np.sort(rng.choice(nrow, 200, replace=False))
+    d_rows = torch.from_numpy(rows.astype(np.int64)).cuda()
+    d_out = torch.zeros(len(rows), dtype=torch.complex128, device="cuda")
+    d_uvw = torch.from_numpy(np.ascontiguousarray(uvw)).cuda()
+    d_img = torch.from_numpy(img).cuda()
+    d_wgt = torch.from_numpy(wgt.astype(np.float32)).cuda()
+    assert lib.curig_explicit_dirty2ms_dev(nrow, n, n, 1.0, f0, d_uvw.data_ptr(), d_img.data_ptr(), d_wgt.data_ptr(),
+                                           1, len(rows), d_rows.data_ptr(), d_out.data_ptr(), -1, None) == 0
+    torch.cuda.synchronize()
+    e4 = O.l2error(d_out.cpu().numpy(), v_lds[rows])
+    del d_img
+    say(f"  dirty2ms: vs rocFFT path {e3:.2e}, vs direct sum {e4:.2e}")
+    assert e1 < 2e-6 and e2 < 3 * eps + 3e-6
+    assert e3 < 2e-6 and e4 < 3 * eps + 3e-6
+
+
 def _lib_plan(lib, nrow, nx, ny, fov, f0, eps, uvw, precision):
     from curig_amd import _lib as L
 

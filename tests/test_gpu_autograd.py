@@ -131,4 +131,6 @@ def test_finite_differences_imaging_loss(dtype):
             fd[q] = (loss(x0 + e).item() - loss(x0 - e).item()) / (2 * h)
     err = np.linalg.norm(fd - grad[pix]) / np.linalg.norm(grad[pix])
     say(f"[autograd] FD relative error {err:.2e}")
-    assert err < (1e-6 if dtype == "fp64" else 2e-2)
+    # FP32: the differences of two FP32-rounded losses, h = 1e-3, measure
+    # about 1.2e-3 here; the bar leaves a factor of four
+    assert err < (1e-6 if dtype == "fp64" else 5e-3)

@@ -106,6 +106,7 @@ struct Geom {
   int lds_col_bytes;  // dynamic LDS of the column-pass workgroup
   int rsplit;         // > 1: rows run as rsplit workgroups of nf1 / rsplit (too long for two per CU)
   int lds_rowp_bytes; // dynamic LDS of a row-part workgroup
+  int colreg;         // 1: FP64 columns of 16384, held in registers by 1024 threads (one column slot each)
 };
 
 // Column-tile width of the intermediate planes (LDS path, both directions):

@@ -40,8 +40,8 @@ template <typename T> FC_HD cpx<T> mul(cpx<T> a, cpx<T> b) {
 template <typename T> FC_HD cpx<T> mulw(cpx<T> a, T c, T s) { return mk(a.x * c - a.y * s, a.x * s + a.y * c); }
 
 // LDS padding: one slot every 16 elements breaks power-of-two strides
-FC_HD int pad(int i) { return i + (i >> 4); }
-FC_HD int padded_size(int n) { return n + (n >> 4) + 1; }
+FC_HD constexpr int pad(int i) { return i + (i >> 4); }
+FC_HD constexpr int padded_size(int n) { return n + (n >> 4) + 1; }
 
 // Padded positions of a butterfly's R elements e0 + r * st as p0 + r * pst when
 // that holds exactly: st a multiple of 16 (each step adds st / 16 pad slots), or

@@ -371,7 +371,7 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
   // workgroup's LDS, with the image at most half the grid on each axis
   g.fft_lds = 0;
   g.rpw = g.cpw = g.nct = g.tstride = g.nslot = g.slot_lg = g.lds_row_bytes = g.lds_col_bytes = g.rsplit =
-      g.lds_rowp_bytes = 0;
+      g.lds_rowp_bytes = g.colreg = 0;
   auto pow2 = [](int n) { return n > 0 && (n & (n - 1)) == 0; };
   const size_t cs = dbl ? 16 : 8;
   const size_t lds_max = tn.lds_bytes > 0 ? (size_t)tn.lds_bytes : size_t(160) << 10;
@@ -383,21 +383,29 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
     // the row kernels' static LDS (bin walk table) comes on top of the row
     const size_t rb = (size_t)(fftc::padded_size(rpw * g.nf1) + fftc::twiddle_entries(g.nf1)) * cs;
     const size_t cb = (size_t)(fftc::padded_size(cpw * g.nf2) + fftc::twiddle_entries(g.nf2)) * cs;
-    if (rb + 2048 <= lds_max && cb <= lds_max && rpw * g.nf1 / 16 <= max_threads &&
-        cpw * g.nf2 / 16 <= max_threads && rpw * g.nf1 >= 16 * 64 && cpw * g.nf2 >= 16) {
+    // a row part of a row split kRowSplit ways (256 threads at most)
+    const size_t rbp = (size_t)(fftc::padded_size(g.nf1 / kRowSplit) + fftc::twiddle_entries(g.nf1)) * cs;
+    const bool row_whole = rb + 2048 <= lds_max && rpw * g.nf1 / 16 <= max_threads;
+    const bool row_split = rpw == 1 && g.nf1 >= 4096 && g.nf1 / kRowSplit <= 16 * 256 && rbp + 2048 <= lds_max / 2;
+    const bool col_whole = cb <= lds_max && cpw * g.nf2 / 16 <= max_threads;
+    // FP64 columns of 16384: held in registers, exchanged through LDS one
+    // component at a time
+    const size_t cbr = colreg_lds_bytes();
+    const bool col_reg = dbl && g.nf2 == 16384 && cbr <= lds_max;
+    if ((row_whole || row_split) && (col_whole || col_reg) && rpw * g.nf1 >= 16 * 64 && cpw * g.nf2 >= 16) {
       g.fft_lds = 1;
       g.rpw = rpw;
-      g.cpw = cpw;
+      g.cpw = col_whole ? cpw : 1;
+      g.colreg = col_whole ? 0 : 1;
       g.nslot = g.N1;
-      g.lds_row_bytes = (int)rb;
-      g.lds_col_bytes = (int)cb;
+      g.lds_row_bytes = row_whole ? (int)rb : 0;
+      g.lds_col_bytes = (int)(col_whole ? cb : cbr);
       // rows too long for two workgroups per CU run split kRowSplit ways
-      if (rpw == 1 && rb + 2048 > lds_max / 2 && g.nf1 >= 4096) {
+      if (!row_whole || (rpw == 1 && rb + 2048 > lds_max / 2 && g.nf1 >= 4096)) {
         g.rsplit = kRowSplit;
         g.nslot = g.nf1 / 2;
         g.slot_lg = 31 - __builtin_clz(g.nf1 / (2 * kRowSplit));
-        g.lds_rowp_bytes =
-            (int)((size_t)(fftc::padded_size(g.nf1 / kRowSplit) + fftc::twiddle_entries(g.nf1)) * cs);
+        g.lds_rowp_bytes = (int)rbp;
       }
       g.nct = (g.nslot + kTile - 1) / kTile;
       g.tstride = g.nf2 * kTile + (int)(kTilePadBytes / cs);
@@ -893,7 +901,7 @@ uint64_t graph_key(bool to_image, const Problem *pbs, int n, const std::vector<C
       << g.beta << g.c << g.dx << g.dy << g.divn << g.zout << g.C << g.D << g.gamma << g.h << g.uscale << g.vscale
       << g.wscale << g.T1 << g.T2 << g.ntx << g.nty << g.ntiles << g.Pb << g.nbatch << g.Pbin << g.nlayer
       << g.nquad << g.fft_lds << g.rpw << g.cpw << g.nct << g.tstride << g.nslot << g.slot_lg << g.lds_row_bytes
-      << g.lds_col_bytes << g.rsplit << g.lds_rowp_bytes;
+      << g.lds_col_bytes << g.rsplit << g.lds_rowp_bytes << g.colreg;
   }
   k << b.dev << b.tabs << b.hist << b.off << b.bsum << b.perm << b.binrank << b.recs << b.taps << b.grid << b.H << b.acc
     << b.vacc << b.tw1 << b.tw2 << b.hc_d << b.hc_f;

@@ -517,7 +517,7 @@ __device__ __forceinline__ void spread_row_part(const Rec<T> rc, const Taps<T, K
 // whose grid index is h mod S).  Several such workgroups fit one CU, so one's
 // memory phases overlap another's FFT.
 template <typename T, int E, int KWM, int S>
-__global__ void __launch_bounds__(MaxThreads<T>::value / S) spread_row_fft_part_kernel(
+__global__ void __launch_bounds__(256) spread_row_fft_part_kernel(
     const Rec<T> *__restrict__ recs, const T *__restrict__ taps, const int *__restrict__ off, cpx<T> *__restrict__ H,
     const cpx<T> *__restrict__ tw, Geom g, int p0) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -630,7 +630,7 @@ __device__ __forceinline__ void interp_row_part(const Rec<T> rc, const Taps<T, K
 // X[S k' + h], so the parts of a row load disjoint columns of H, run m-point
 // FFTs and add their shares to each record's accumulator.
 template <typename T, int E, int KWM, int S>
-__global__ void __launch_bounds__(MaxThreads<T>::value / S) row_ifft_interp_part_kernel(
+__global__ void __launch_bounds__(256) row_ifft_interp_part_kernel(
     const cpx<T> *__restrict__ H, const Rec<T> *__restrict__ recs, const T *__restrict__ taps,
     const int *__restrict__ off,
     double2 *__restrict__ vacc, const cpx<T> *__restrict__ tw, Geom g, int p0, int sign) {
@@ -847,6 +847,225 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) col_inv_kernel(const cpx
 }
 
 // --------------------------------------------------------------------------
+// FP64 columns of 16384 (256 KiB, more than a CU's LDS): one workgroup of 1024
+// threads per column slot keeps the column in registers, 16 elements per
+// thread, and runs the four Stockham stages (radices 16, 16, 16, 4) with the
+// exchanges between stages through a one-component LDS buffer (16384 doubles
+// plus pad), the real parts first, then the imaginary parts.  Thread j holds
+// x[j + 1024 r] in v[r] before each radix-16 stage and, after the radix-4
+// stage, X[j + 1024 b + 4096 r] in v[4 b + r] (b, r = 0..3).
+// --------------------------------------------------------------------------
+namespace colr {
+constexpr int N = 16384, NT = 1024;
+// doubles of the exchange buffer (rounded up to 16 bytes: the twiddles follow)
+constexpr int kBuf = fftc::padded_size(N) + 1;
+
+// v through LDS, one component at a time: stored at padded position st(r),
+// reloaded from ld(r) (each a base plus a constant step, one address register)
+template <typename T, typename St, typename Ld>
+__device__ __forceinline__ void exchange(cpx<T> *v, T *buf, St st, Ld ld) {
+#pragma unroll
+  for (int r = 0; r < 16; ++r) buf[st(r)] = v[r].x;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; ++r) v[r].x = buf[ld(r)];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; ++r) buf[st(r)] = v[r].y;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; ++r) v[r].y = buf[ld(r)];
+  __syncthreads();
+}
+
+// u[r] *= w^r (sign applied to w) for r = 1..R-1, w = tv[base]: w, w^2, w^4, w^8
+// from the table, each other power formed next to its use (few live registers)
+template <typename T, int R, typename Tw>
+__device__ __forceinline__ void apply_twiddles(cpx<T> *u, const Tw &tv, int base, int sign) {
+  const int n = N - 1;
+  cpx<T> p[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if ((1 << i) < R) {
+      const cpx<T> w = tv[(base << i) & n];
+      p[i] = fftc::mk(w.x, (T)sign * w.y);
+    }
+  }
+#pragma unroll
+  for (int r = 1; r < R; ++r) {
+    // w^r as the product of the powers of two in r
+    cpx<T> w = p[__builtin_ctz(r)];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (((r >> i) & 1) && i != __builtin_ctz(r)) w = fftc::mul(w, p[i]);
+    u[r] = fftc::mul(u[r], w);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// the thread id through an empty asm: the stage addresses are formed where they
+// are used (hoisted out of the plane loop they would not fit the registers)
+__device__ __forceinline__ int opaque_tid() {
+  int j = threadIdx.x;
+  __asm__ volatile("" : "+v"(j));
+  return j;
+}
+
+// 16384-point FFT of the column held as described above (sign +1 forward in the
+// sense of the row pass, -1 inverse); tv: twiddles e^{2 pi i t / 16384}.
+// Padded positions: pad(j + 1024 r) = pad(j) + 1088 r, pad(16 j + r) = 17 j + r,
+// pad(d + 16 r) = pad(d) + 17 r and pad(d + 256 r) = pad(d) + 272 r for d mod 16 = d mod 16 < 16.
+template <typename T, typename Tw>
+__device__ __forceinline__ void fft(cpx<T> *v, T *buf, int sign, const Tw &tv) {
+  // stage 1 (sub-transform size 1)
+  fftc::dft_small<T, 16>(v, sign);
+  __builtin_amdgcn_sched_barrier(0);
+  {
+    const int j = opaque_tid(), s0 = 17 * j, l0 = fftc::pad(j);
+    exchange(v, buf, [&](int r) { return s0 + r; }, [&](int r) { return l0 + 1088 * r; });
+  }
+  // stage 2 (size 16): twiddles W256^(k r), k = j mod 16
+  {
+    const int j = opaque_tid(), k = j & 15;
+    apply_twiddles<T, 16>(v, tv, 64 * k, sign);
+    fftc::dft_small<T, 16>(v, sign);
+    __builtin_amdgcn_sched_barrier(0);
+    const int j2 = opaque_tid(), k2 = j2 & 15, s0 = fftc::pad(16 * (j2 - k2) + k2), l0 = fftc::pad(j2);
+    exchange(v, buf, [&](int r) { return s0 + 17 * r; }, [&](int r) { return l0 + 1088 * r; });
+  }
+  // stage 3 (size 256): twiddles W4096^(k r), k = j mod 256; reloaded as the
+  // radix-4 inputs j + 1024 b + 4096 r
+  {
+    const int j = opaque_tid(), k = j & 255;
+    apply_twiddles<T, 16>(v, tv, 4 * k, sign);
+    fftc::dft_small<T, 16>(v, sign);
+    __builtin_amdgcn_sched_barrier(0);
+    const int j2 = opaque_tid(), k2 = j2 & 255, s0 = fftc::pad(16 * (j2 - k2) + k2), l0 = fftc::pad(j2);
+    exchange(v, buf, [&](int r) { return s0 + 272 * r; },
+             [&](int e) { return l0 + 1088 * (e >> 2) + 4352 * (e & 3); });
+  }
+  // stage 4 (size 4096, radix 4): butterfly beta = j + 1024 b, twiddles W16384^(beta r)
+  const int j = opaque_tid();
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    apply_twiddles<T, 4>(v + 4 * b, tv, j + 1024 * b, sign);
+    fftc::dft_small<T, 4>(v + 4 * b, sign);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+}  // namespace colr
+
+// ms2dirty column pass for FP64 columns of 16384 (col_fwd_kernel's work: column
+// FFT, then the N2 image rows accumulated with their w phases in FP64).  Of the
+// outputs X[j + 1024 b + 4096 r] only r = 0 and r = 3 can be image rows (grid
+// indices within N2 / 2 <= nf2 / 4 of 0).
+template <typename T>
+__global__ void __launch_bounds__(colr::NT) col_fwd_reg_kernel(const cpx<T> *__restrict__ H,
+                                                               double2 *__restrict__ accT,
+                                                               const cpx<T> *__restrict__ tw, Geom g, int p0,
+                                                               int nb, int first) {
+  using colr::N;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *buf = reinterpret_cast<T *>(smem_raw);
+  const int j = threadIdx.x, N2 = g.N2;
+  const int sl = xcd_group(blockIdx.x, gridDim.x);  // column slot
+  const int col = slot_column(sl, g);
+  if (col < 0) return;
+  const auto tv = lds_twiddles<T>(reinterpret_cast<cpx<T> *>(buf + colr::kBuf), tw, N);
+  // output q of this thread: X[j + 1024 (q mod 4) + 4096 r], r = 0 (q < 4) or 3;
+  // its image row, or -1 (rows and w factors are recomputed per plane: held
+  // across the plane loop they would not fit the registers beside the column)
+  auto out_row = [&](int q) { return image_index(colr::opaque_tid() + 1024 * (q & 3) + (q < 4 ? 0 : 3 * 4096), N2, N); };
+  double ar[8], ai[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) ar[q] = ai[q] = 0.0;
+  __syncthreads();
+  const cpx<T> *hs = H + h_slot(g, sl);
+  for (int k = 0; k < nb; ++k) {
+    const cpx<T> *hk = hs + h_plane(g, k);
+    cpx<T> v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = hk[(size_t)(j + 1024 * r) << kTileLog2];
+    colr::fft(v, buf, +1, tv);
+    const double m = (double)(p0 + k - g.P / 2);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int row = out_row(q);
+      if (row >= 0) {
+        const cpx<T> f = v[4 * (q & 3) + (q < 4 ? 0 : 3)];
+        double sn, cs;
+        plane_phase<T>(pixel_zp(row, col, g) * m, &sn, &cs);
+        const double fr = (double)f.x, fi = (double)f.y;
+        ar[q] += fr * cs - fi * sn;
+        ai[q] += fr * sn + fi * cs;
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int row = out_row(q);
+    if (row >= 0) {
+      const size_t a = (size_t)col * N2 + row;
+      double2 r = make_double2(ar[q], ai[q]);
+      if (!first) {
+        const double2 o = accT[a];
+        r.x += o.x;
+        r.y += o.y;
+      }
+      accT[a] = r;
+    }
+  }
+}
+
+// dirty2ms column pass for FP64 columns of 16384 (col_inv_kernel's work: the
+// image rows of the column with their w phases, zeros elsewhere, inverse FFT,
+// the full column written to H).  The inputs x[j + 1024 r] are image rows for
+// r = 0..3 and 12..15 only.
+template <typename T>
+__global__ void __launch_bounds__(colr::NT) col_inv_reg_kernel(const cpx<T> *__restrict__ gT,
+                                                               cpx<T> *__restrict__ H,
+                                                               const cpx<T> *__restrict__ tw, Geom g, int p0,
+                                                               int nb) {
+  using colr::N;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *buf = reinterpret_cast<T *>(smem_raw);
+  const int j = threadIdx.x, N2 = g.N2;
+  const int sl = xcd_group(blockIdx.x, gridDim.x);  // column slot
+  const int col = slot_column(sl, g);
+  if (col < 0) return;
+  const auto tv = lds_twiddles<T>(reinterpret_cast<cpx<T> *>(buf + colr::kBuf), tw, N);
+  // input q of this thread: x[j + 1024 r], r = q (q < 4) or q + 8; its image
+  // row, or -1 (recomputed per plane, as in col_fwd_reg_kernel)
+  auto in_row = [&](int q) { return image_index(colr::opaque_tid() + 1024 * (q < 4 ? q : q + 8), N2, N); };
+  __syncthreads();
+  const cpx<T> *gc = gT + (size_t)col * N2;
+  cpx<T> *hs = H + h_slot(g, sl);
+  for (int k = 0; k < nb; ++k) {
+    const double m = (double)(p0 + k - g.P / 2);
+    cpx<T> v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = fftc::mk((T)0, (T)0);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int row = in_row(q);
+      if (row >= 0) {
+        const cpx<T> gv = gc[row];
+        double sn, cs;
+        plane_phase<T>(-pixel_zp(row, col, g) * m, &sn, &cs);
+        const double gr = (double)gv.x, gi = (double)gv.y;
+        v[q < 4 ? q : q + 8] = fftc::mk((T)(gr * cs - gi * sn), (T)(gr * sn + gi * cs));
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    colr::fft(v, buf, -1, tv);
+    cpx<T> *hk = hs + h_plane(g, k);
+#pragma unroll
+    for (int e = 0; e < 16; ++e) hk[(size_t)(j + 1024 * (e >> 2) + 4096 * (e & 3)) << kTileLog2] = v[e];
+  }
+}
+
+// --------------------------------------------------------------------------
 // launchers (the LDS path takes kernel widths up to 16: the plan sends wider
 // kernels to the full-plane path)
 // --------------------------------------------------------------------------
@@ -926,6 +1145,12 @@ int allow_row_interp(const Geom &g) {
 template <typename T>
 void lds_cols_forward(const Geom &g, int p0, int nb, const cpx<T> *H, double2 *accT, const cpx<T> *tw2, int first,
                       hipStream_t st) {
+  if constexpr (sizeof(T) == 8) {
+    if (g.colreg) {
+      col_fwd_reg_kernel<T><<<col_groups(g), colr::NT, g.lds_col_bytes, st>>>(H, accT, tw2, g, p0, nb, first);
+      return;
+    }
+  }
   col_fwd_kernel<T, 16><<<col_groups(g), g.cpw * g.nf2 / 16, g.lds_col_bytes, st>>>(H, accT, tw2, g, p0, nb, first);
 }
 
@@ -933,11 +1158,24 @@ void lds_cols_forward(const Geom &g, int p0, int nb, const cpx<T> *H, double2 *a
 template <typename T>
 void lds_cols_backward(const Geom &g, int p0, int nb, const cpx<T> *gT, cpx<T> *H, const cpx<T> *tw2,
                        hipStream_t st) {
+  if constexpr (sizeof(T) == 8) {
+    if (g.colreg) {
+      col_inv_reg_kernel<T><<<col_groups(g), colr::NT, g.lds_col_bytes, st>>>(gT, H, tw2, g, p0, nb);
+      return;
+    }
+  }
   col_inv_kernel<T, 16><<<col_groups(g), g.cpw * g.nf2 / 16, g.lds_col_bytes, st>>>(gT, H, tw2, g, p0, nb);
 }
 
 template <typename T>
 int allow_cols(const Geom &g) {
+  if constexpr (sizeof(T) == 8) {
+    if (g.colreg) {
+      CURIG_TRY(allow_lds(col_fwd_reg_kernel<T>, g.lds_col_bytes));
+      CURIG_TRY(allow_lds(col_inv_reg_kernel<T>, g.lds_col_bytes));
+      return kOk;
+    }
+  }
   CURIG_TRY(allow_lds(col_fwd_kernel<T, 16>, g.lds_col_bytes));
   CURIG_TRY(allow_lds(col_inv_kernel<T, 16>, g.lds_col_bytes));
   return kOk;

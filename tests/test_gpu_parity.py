@@ -404,11 +404,13 @@ def test_lds_fft_path_equals_rocfft_path(precision):
     assert O.l2error(v_roc, v_lds) < tol
 
 
-@pytest.mark.parametrize("precision,n", [("fp64", 4096), ("fp32", 8192), ("fp32", 8191)])
+@pytest.mark.parametrize("precision,n", [("fp64", 4096), ("fp64", 8192), ("fp32", 8192), ("fp32", 8191)])
 def test_split_rows_equal_rocfft_path(precision, n):
-    """Grids whose rows exceed half a CU's LDS (16384-point FP32, 8192-point
-    FP64) run each row as several shorter workgroups (four by default); the result equals the
-    rocFFT path and, on a few visibilities, the FP64 direct sum."""
+    """Grids whose rows exceed half a CU's LDS (16384-point FP32, 8192- and
+    16384-point FP64) run each row as several shorter workgroups (four by
+    default), and FP64 columns of 16384 (more than a CU's LDS) run in registers
+    with one-component LDS exchanges; the result equals the rocFFT path and, on
+    a few visibilities, the FP64 direct sum."""
     lib = _lib()
     say(f"[split rows] {precision} {n}x{n}")
     rng = np.random.default_rng(7)
@@ -425,7 +427,7 @@ def test_split_rows_equal_rocfft_path(precision, n):
     fn_v = gpu_dirty2ms if precision == "fp64" else (lambda *a: gpu_f32(False, *a))
     eps = 1e-6
     info = _lib_plan(lib, len(uvw), n, n, 1.0, f0, eps, uvw, precision)
-    assert info["fft_lds"] == 1
+    assert info["fft_lds"] == 1 and info["rsplit"] == 4
     d_lds = fn_d(uvw, vis, wgt, n, n, 1.0, f0, eps)
     v_lds = fn_v(uvw, img, wgt, n, n, 1.0, f0, eps)
     assert lib.curig_set_fft_path(1) == 0

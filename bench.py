@@ -783,8 +783,10 @@ def main():
     launches = kb["launches"]
     mean = lambda key, d: sum(c[key] for c in calls[d]) / len(calls[d]) * 1e-3
     lds = bool(stats_by_ch[0]["fft_lds"])
-    part = "_part" if stats_by_ch[0]["rsplit"] > 0 else ""  # rows split over several workgroups
-    kernels = {f"spread_row_fft{part}_kernel" if lds else "spread_tile_kernel": (kb["grid"], mean("t_grid", "m2d"))}
+    split = stats_by_ch[0]["rsplit"] > 0  # rows split over several workgroups
+    part = "_part" if split else ""
+    spread = ("spread_row_part_planes_kernel" if split else "spread_row_fft_kernel") if lds else "spread_tile_kernel"
+    kernels = {spread: (kb["grid"], mean("t_grid", "m2d"))}
     if lds:
         kernels["col_fwd_kernel"] = (kb["col_fwd"], mean("t_fft", "m2d"))
         kernels["col_inv_kernel"] = (kb["col_inv"], mean("t_fft", "d2m"))

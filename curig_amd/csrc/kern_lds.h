@@ -86,6 +86,65 @@ __device__ __forceinline__ int walk_record(const WalkTable &wt, int t) {
   return wt.start[lo] + (t - wt.pre[lo]);
 }
 
+// Bin offsets of a walk table (rows [row0, row0 + nrows)) loaded ahead by the 64 lanes of wave 0
+// (bins k = lane and lane + 64), completed into a WalkTable later
+// (walk_finish), so the load's latency hides behind other work.
+struct WalkAhead {
+  int st[2], en[2];
+  int nb;
+};
+
+__device__ __forceinline__ void walk_issue(WalkAhead &wa, const int *__restrict__ off, const Geom &g, int p,
+                                           int row0, int nrows = 1) {
+  const int lane = threadIdx.x & 63;
+  const int al = imod(p - g.kww + 1, g.P), as = imod(row0 - g.kw + 1, g.nf2);
+  const int nl = tiles_count(al, g.kww, g.P, g.Pbin, g.nlayer);
+  const int ns = tiles_count(as, nrows + g.kw - 1, g.nf2, g.T2, g.nty);
+  int nb = nl * ns;
+  if (nb > kMaxWalk) nb = kMaxWalk;  // the plan keeps nb within kMaxWalk
+  wa.nb = nb;
+  const int l0 = al / g.Pbin, s0 = as / g.T2;
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int k = r * 64 + lane;
+    wa.st[r] = wa.en[r] = 0;
+    if (k < nb) {
+      const int j = k / ns, i = k - j * ns;
+      int layer = l0 + j, strip = s0 + i;
+      if (layer >= g.nlayer) layer -= g.nlayer;
+      if (strip >= g.nty) strip -= g.nty;
+      const int b = layer * g.ntiles + strip;
+      wa.st[r] = off[b];
+      wa.en[r] = off[b + 1];
+    }
+  }
+}
+
+__device__ __forceinline__ void walk_finish(WalkTable &wt, const WalkAhead &wa) {
+  const int lane = threadIdx.x & 63;
+  int carry = 0;
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int k = r * 64 + lane;
+    const int len = k < wa.nb ? wa.en[r] - wa.st[r] : 0;
+    int x = len;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int y = __shfl_up(x, d, 64);
+      if (lane >= d) x += y;
+    }
+    if (k < wa.nb) {
+      wt.start[k] = wa.st[r];
+      wt.pre[k] = carry + x - len;
+    }
+    carry += __shfl(x, 63, 64);
+  }
+  if (lane == 0) {
+    wt.nb = wa.nb;
+    wt.total = carry;
+  }
+}
+
 // Copies a two-level twiddle table (hi | lo, fftc::twiddle_entries(n) entries)
 // from global memory to LDS at dst; the caller's next barrier publishes it.
 template <typename T>
@@ -412,6 +471,7 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) spread_row_fft_kernel(co
   }
 }
 
+
 // dirty2ms: row pass of the column-transformed planes (H as col_inv_kernel
 // writes it; each row expanded to nf1 cells, zero outside the image columns),
 // inverse row FFT in LDS, then interpolation of every record touching the workgroup's rows of
@@ -510,78 +570,100 @@ __device__ __forceinline__ void spread_row_part(const Rec<T> rc, const Taps<T, K
   }
 }
 
+
 // ms2dirty row pass for rows too long for two workgroups per CU: one workgroup
 // per part h of a row split S ways (the row transform's outputs S k + h, see
 // spread_row_part), each spreading the row's visibilities into an m-point LDS
 // buffer, transforming it and writing its share of the N1 image columns (those
 // whose grid index is h mod S).  Several such workgroups fit one CU, so one's
-// memory phases overlap another's FFT.
+// memory phases overlap another's FFT.  The workgroup keeps its (row, part)
+// and walks the batch's nb planes, as the column kernels do: while plane p is
+// transformed, the walk table of plane p + 1 is complete and each thread's
+// first record of it is in flight, and the twiddle table and the part's
+// factors are loaded once (c3: 23.8 against 24.3 ms per 32-plane launch for
+// one workgroup per (row, part, plane)).
 template <typename T, int E, int KWM, int S>
-__global__ void __launch_bounds__(256) spread_row_fft_part_kernel(
+__global__ void __launch_bounds__(256) spread_row_part_planes_kernel(
     const Rec<T> *__restrict__ recs, const T *__restrict__ taps, const int *__restrict__ off, cpx<T> *__restrict__ H,
-    const cpx<T> *__restrict__ tw, Geom g, int p0) {
+    const cpx<T> *__restrict__ tw, Geom g, int p0, int nb) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  __shared__ WalkTable wt;
+  __shared__ WalkTable wt[2];
+  __shared__ cpx<T> whs[KWM];
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
   T *ldsf = reinterpret_cast<T *>(smem_raw);
-  const int nf1 = g.nf1, m = nf1 / S, nt = blockDim.x, sign = +1;
+  const int nf1 = g.nf1, m = nf1 / S, nt = blockDim.x, sign = +1, j = threadIdx.x;
   // the parts of four consecutive rows (whose chunks share lines of H) are
   // consecutive items of one XCD; the XCDs take such blocks in turn
   const int item = xcd_interleave(blockIdx.x, gridDim.x, 4 * S);
-  const int row0 = item / S, h = item % S, pl = blockIdx.y, p = p0 + pl;
+  const int row0 = item / S, h = item % S;
   const int ntot = fftc::padded_size(m);
   const auto tv = lds_twiddles<T>(lds + ntot, tw, nf1);
-  if (threadIdx.x < 64) walk_table_build(wt, off, g, p, row0, 1);
-  for (int i = threadIdx.x; i < ntot; i += nt) lds[i] = fftc::mk((T)0, (T)0);
+  if (j < 64) walk_table_build(wt[0], off, g, p0, row0, 1);
+  for (int i = j; i < ntot; i += nt) lds[i] = fftc::mk((T)0, (T)0);
+  if (j < KWM) {
+    // w^(sign h i) for the taps i (part h's decimation factor), from the
+    // global twiddle table (the LDS copy is being written)
+    const int lb = fftc::twiddle_split(nf1);
+    const fftc::TwoLevel<T, const cpx<T> *> tg{tw, tw + (nf1 >> lb), lb};
+    const cpx<T> w = tg[h * j];
+    whs[j] = fftc::mk(w.x, (T)sign * w.y);
+  }
   __syncthreads();
-  const int total = wt.total;
   // this part's image columns: grid_index(col) = col - N1/2 (mod nf1) = h (mod S)
   const int N1 = g.N1, h1 = N1 / 2, c0 = (h + h1) % S, nout = (N1 - c0 + S - 1) / S;
-  cpx<T> *hb = H + h_plane(g, pl) + row0 * kTile;
   const int hs = h << g.slot_lg;  // this part's first column slot
-  if (total == 0) {
-    for (int i = threadIdx.x; i < nout; i += nt) {
-      const int sl = hs + i;
-      hb[h_slot(g, sl)] = fftc::mk((T)0, (T)0);
+  // this thread's first record of plane p0
+  Rec<T> r0;
+  Taps<T, KWM> q0;
+  if (j < wt[0].total) {
+    const int idx = walk_record(wt[0], j);
+    r0 = recs[idx];
+    load_taps<T, KWM>(taps, idx, q0);
+  }
+  int cur = 0;
+  for (int pl = 0; pl < nb; ++pl, cur ^= 1) {
+    const int p = p0 + pl, total = wt[cur].total;
+    const bool more = pl + 1 < nb;
+    WalkAhead wa;
+    if (j < 64 && more) walk_issue(wa, off, g, p + 1, row0);
+    if (j < total) spread_row_part<T, KWM>(r0, q0, ldsf, tv, whs, g, p, row0, h, m - 1, sign);
+    for (int t = j + nt; t < total; t += nt) {
+      const int ia = walk_record(wt[cur], t);
+      Taps<T, KWM> ta;
+      load_taps<T, KWM>(taps, ia, ta);
+      spread_row_part<T, KWM>(recs[ia], ta, ldsf, tv, whs, g, p, row0, h, m - 1, sign);
     }
-    return;
-  }
-  // w^(sign h i) for the taps i (part h's decimation factor), from the LDS twiddles
-  cpx<T> wh[KWM];
-#pragma unroll
-  for (int i = 0; i < KWM; ++i) {
-    const cpx<T> w = tv[h * i];
-    wh[i] = fftc::mk(w.x, (T)sign * w.y);
-  }
-  for (int t0 = threadIdx.x; t0 < total; t0 += nt) {
-    const int ia = walk_record(wt, t0);
-    const Rec<T> ra = recs[ia];
-    Taps<T, KWM> ta;
-    load_taps<T, KWM>(taps, ia, ta);
-    spread_row_part<T, KWM>(ra, ta, ldsf, tv, wh, g, p, row0, h, m - 1, sign);
-  }
-  __syncthreads();
-  lds_fft_part<T, E, 16, S>(lds, 31 - __clz(m), sign, tv);
-  int tid = threadIdx.x;
-  __asm__ volatile("" : "+v"(tid));
-  constexpr int NO = E / 2;  // nout <= m / 2 = (E / 2) nt, since N1 <= nf1 / 2
-  cpx<T> o[NO];
-#pragma unroll
-  for (int e = 0; e < NO; ++e) {
-    const int i = tid + e * nt;
-    if (i < nout) {
-      const int col = S * i + c0;
-      const int k1 = col >= h1 ? col - h1 : nf1 + col - h1;
-      o[e] = lds[fftc::pad(k1 / S)];
+    if (j < 64 && more) walk_finish(wt[cur ^ 1], wa);
+    __syncthreads();  // spread done; the next plane's walk table is complete
+    // the next plane's first record, in flight during this plane's transform
+    if (more && j < wt[cur ^ 1].total) {
+      const int idx = walk_record(wt[cur ^ 1], j);
+      r0 = recs[idx];
+      load_taps<T, KWM>(taps, idx, q0);
     }
-  }
-#pragma unroll
-  for (int e = 0; e < NO; ++e) {
-    const int i = tid + e * nt;
-    if (i < nout) {
-      const int sl = hs + i;
-      hb[h_slot(g, sl)] = o[e];
+    cpx<T> *hb = H + h_plane(g, pl) + row0 * kTile;
+    if (total == 0) {
+      // no visibility reaches this part of this row and plane: its outputs are
+      // zero (and the LDS block still is)
+      for (int i = j; i < nout; i += nt) hb[h_slot(g, hs + i)] = fftc::mk((T)0, (T)0);
+      continue;
     }
+    lds_fft_part<T, E, 16, S>(lds, 31 - __clz(m), sign, tv);
+    int tid = threadIdx.x;
+    __asm__ volatile("" : "+v"(tid));
+    // nout <= m / 2 = (E / 2) nt, since N1 <= nf1 / 2
+#pragma unroll
+    for (int e = 0; e < E / 2; ++e) {
+      const int i = tid + e * nt;
+      if (i < nout) {
+        const int col = S * i + c0;
+        const int k1 = col >= h1 ? col - h1 : nf1 + col - h1;
+        hb[h_slot(g, hs + i)] = lds[fftc::pad(k1 / S)];
+      }
+    }
+    __syncthreads();  // every output read: clear the block for the next plane
+    for (int i = tid; i < ntot; i += nt) lds[i] = fftc::mk((T)0, (T)0);
+    __syncthreads();  // cleared before the next plane's spread
   }
 }
 
@@ -1082,12 +1164,11 @@ template <typename T>
 void lds_spread_rows(const Rec<T> *recs, const T *taps, const int *off, const Geom &g, int p0, int nb, cpx<T> *H,
                      const cpx<T> *tw1, hipStream_t st) {
   if (g.rsplit == kRowSplit) {
-    const dim3 grd(kRowSplit * g.nf2, nb);
-    const int nt = g.nf1 / (16 * kRowSplit), lb = g.lds_rowp_bytes;
+    const int nt = g.nf1 / (16 * kRowSplit), lb = g.lds_rowp_bytes, grd = kRowSplit * g.nf2;
     if (g.kw <= 8)
-      spread_row_fft_part_kernel<T, 16, 8, kRowSplit><<<grd, nt, lb, st>>>(recs, taps, off, H, tw1, g, p0);
+      spread_row_part_planes_kernel<T, 16, 8, kRowSplit><<<grd, nt, lb, st>>>(recs, taps, off, H, tw1, g, p0, nb);
     else
-      spread_row_fft_part_kernel<T, 16, 16, kRowSplit><<<grd, nt, lb, st>>>(recs, taps, off, H, tw1, g, p0);
+      spread_row_part_planes_kernel<T, 16, 16, kRowSplit><<<grd, nt, lb, st>>>(recs, taps, off, H, tw1, g, p0, nb);
     return;
   }
   const dim3 grd(g.nf2 / g.rpw, nb);
@@ -1103,8 +1184,8 @@ int allow_spread_rows(const Geom &g) {
   CURIG_TRY(allow_lds(spread_row_fft_kernel<T, 16, 8>, g.lds_row_bytes));
   CURIG_TRY(allow_lds(spread_row_fft_kernel<T, 16, 16>, g.lds_row_bytes));
   if (g.rsplit == kRowSplit) {
-    CURIG_TRY((allow_lds(spread_row_fft_part_kernel<T, 16, 8, kRowSplit>, g.lds_rowp_bytes)));
-    CURIG_TRY((allow_lds(spread_row_fft_part_kernel<T, 16, 16, kRowSplit>, g.lds_rowp_bytes)));
+    CURIG_TRY((allow_lds(spread_row_part_planes_kernel<T, 16, 8, kRowSplit>, g.lds_rowp_bytes)));
+    CURIG_TRY((allow_lds(spread_row_part_planes_kernel<T, 16, 16, kRowSplit>, g.lds_rowp_bytes)));
   }
   return kOk;
 }

@@ -724,12 +724,24 @@ __global__ void __launch_bounds__(256) row_ifft_interp_part_kernel(
   const int row0 = item / S, h = item % S, pl = blockIdx.y, p = p0 + pl;
   const int lgm = 31 - __clz(m);
   const cpx<T> *hb = H + h_plane(g, pl) + row0 * kTile;
+  // Input e of this thread is X[S i + h], i = tid + e nt (< m = E nt).  With
+  // N1 <= nf1 / 2 only the first and last quarter of i can be image columns
+  // (grid indices within N1 / 2 of 0); their column slots are
+  // h nf1 / (2 S) + i + q (first quarter) and the same minus m (last), with
+  // q = (h + N1 / 2) / S, so their addresses step by (nt / kTile) tiles.
+  const int tid = threadIdx.x, h1 = N1 / 2, q = (h + h1) / S;
+  const int k0 = S * tid + h + h1;  // column of e = 0 before the fold
+  const cpx<T> *a0 = hb + h_slot(g, (h << g.slot_lg) + tid + q);
+  const long estep = (long)(nt >> kTileLog2) * g.tstride, mstep = (long)(m >> kTileLog2) * g.tstride;
   cpx<T> v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const int i = threadIdx.x + e * nt;  // < m = E * nt
-    const int col = image_index(S * i + h, N1, nf1), sl = (h << g.slot_lg) + col / S;
-    v[e] = col >= 0 ? hb[h_slot(g, sl)] : fftc::mk((T)0, (T)0);
+    v[e] = fftc::mk((T)0, (T)0);
+    if (e < E / 4) {
+      if (k0 + S * nt * e < N1) v[e] = a0[e * estep];
+    } else if (e >= 3 * E / 4) {
+      if (k0 + S * nt * e >= nf1) v[e] = a0[e * estep - mstep];
+    }
   }
   const auto tv = lds_twiddles<T>(lds + fftc::padded_size(m), tw, nf1);
   if (threadIdx.x < 64) walk_table_build(wt, off, g, p, row0, 1);

@@ -220,6 +220,66 @@ __device__ __forceinline__ void lds_fft_part(cpx<T> *buf, int lgm, int sign, con
   }
 }
 
+// lds_fft_part with the first radix-16 stage on the thread's registers: v[e]
+// holds x[tid + nt e] (m = 16 nt, the first Stockham stage's butterfly), so
+// the inputs need not pass through LDS before it.
+template <typename T, int E, int MAXR, int N, int S, typename Tw>
+__device__ __forceinline__ void part_stages_from_regs(cpx<T> *buf, cpx<T> *v, int sign, const Tw &tw) {
+  static_assert(E == 16 && MAXR == 16, "one radix-16 butterfly per thread");
+  fftc::dft_small<T, 16>(v, sign);
+  const fftc::Layout L{N, 1, N, 1, false};
+  int tid = threadIdx.x;
+  __asm__ volatile("" : "+v"(tid));
+  fftc::stage_store<T, E, 16>(buf, v, L, 1, tid, blockDim.x);
+  __syncthreads();
+  fixed_stages<T, E, MAXR, N, 16, S>(buf, 1, sign, tw);
+}
+template <typename T, int E, int MAXR, int S, typename Tw>
+__device__ __forceinline__ void lds_fft_part_from_regs(cpx<T> *buf, cpx<T> *v, int lgm, int sign, const Tw &tw) {
+  switch (lgm) {
+    case 11: part_stages_from_regs<T, E, MAXR, 2048, S>(buf, v, sign, tw); break;
+    case 12: part_stages_from_regs<T, E, MAXR, 4096, S>(buf, v, sign, tw); break;
+    default: part_stages_from_regs<T, E, MAXR, 8192, S>(buf, v, sign, tw); break;
+  }
+}
+
+// A 4096-point part (256 threads, three radix-16 stages) whose last stage ends
+// in registers: thread j returns X[j + 256 r] in v[r], having cleared the LDS
+// elements it read last (every element of the part, for the next spread).
+template <typename T, int S, typename Tw>
+__device__ __forceinline__ void fft4096_part_to_regs(cpx<T> *buf, cpx<T> *v, int sign, const Tw &tw) {
+  constexpr int N = 4096, NT = 256;
+  const fftc::Layout L{N, 1, N, 1, false};
+  int tid = threadIdx.x;
+  __asm__ volatile("" : "+v"(tid));
+  fftc::stage_load<T, 16, 16>(buf, v, L, 1, sign, tw, N * S, tid, NT);
+  __syncthreads();
+  fftc::stage_store<T, 16, 16>(buf, v, L, 1, tid, NT);
+  __syncthreads();
+  int tid2 = threadIdx.x;
+  __asm__ volatile("" : "+v"(tid2));
+  fftc::stage_load<T, 16, 16>(buf, v, L, 16, sign, tw, N * S, tid2, NT);
+  __syncthreads();
+  fftc::stage_store<T, 16, 16>(buf, v, L, 16, tid2, NT);
+  __syncthreads();
+  int tid3 = threadIdx.x;
+  __asm__ volatile("" : "+v"(tid3));
+  // last stage: butterfly j reads x[j + 256 r] (pad(j) + 272 r) and its
+  // outputs are X[j + 256 r]
+  cpx<T> *own = buf + fftc::pad(tid3);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) v[r] = own[272 * r];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) own[272 * r] = fftc::mk((T)0, (T)0);
+  {
+    cpx<T> w[16];
+    fftc::twiddle_powers<T, 16>(w, tw, tid3 * (N * S / (256 * 16)));
+#pragma unroll
+    for (int r = 1; r < 16; ++r) v[r] = fftc::mulw(v[r], w[r].x, (T)sign * w[r].y);
+  }
+  fftc::dft_small<T, 16>(v, sign);
+}
+
 // FFT of `count` contiguous sequences of length n = 2^lg (6 <= lg <= 14) in LDS.
 template <typename T, int E, int MAXR, typename Tw>
 __device__ __forceinline__ void lds_fft_rows(cpx<T> *buf, int lg, int count, int sign, const Tw &tw) {
@@ -648,22 +708,43 @@ __global__ void __launch_bounds__(256) spread_row_part_planes_kernel(
       for (int i = j; i < nout; i += nt) hb[h_slot(g, hs + i)] = fftc::mk((T)0, (T)0);
       continue;
     }
-    lds_fft_part<T, E, 16, S>(lds, 31 - __clz(m), sign, tv);
-    int tid = threadIdx.x;
-    __asm__ volatile("" : "+v"(tid));
-    // nout <= m / 2 = (E / 2) nt, since N1 <= nf1 / 2
+    if (sizeof(T) == 4 && E == 16 && m == 4096) {
+      // last stage in registers (FP32: FP64 would hold 32 more registers): output X[j + 256 r] (part index) is image
+      // column S (j + 256 r) + h + N1 / 2 (folded by nf1), which can be an
+      // image column for r < 4 and r >= 12 only; its slot steps by 64 tiles
+      // per r (as in row_ifft_interp_part_kernel's inputs)
+      cpx<T> v[16];
+      fft4096_part_to_regs<T, S>(lds, v, sign, tv);
+      const int tid = threadIdx.x, q = (h + h1) / S, k0 = S * tid + h + h1;
+      cpx<T> *a0 = hb + h_slot(g, hs + tid + q);
+      const long estep = (long)(256 >> kTileLog2) * g.tstride, mstep = (long)(4096 >> kTileLog2) * g.tstride;
 #pragma unroll
-    for (int e = 0; e < E / 2; ++e) {
-      const int i = tid + e * nt;
-      if (i < nout) {
-        const int col = S * i + c0;
-        const int k1 = col >= h1 ? col - h1 : nf1 + col - h1;
-        hb[h_slot(g, hs + i)] = lds[fftc::pad(k1 / S)];
+      for (int r = 0; r < 16; ++r) {
+        if (r < 4) {
+          if (k0 + S * 256 * r < N1) a0[r * estep] = v[r];
+        } else if (r >= 12) {
+          if (k0 + S * 256 * r >= nf1) a0[r * estep - mstep] = v[r];
+        }
       }
+      __syncthreads();  // the block is clear before the next plane's spread
+    } else {
+      lds_fft_part<T, E, 16, S>(lds, 31 - __clz(m), sign, tv);
+      int tid = threadIdx.x;
+      __asm__ volatile("" : "+v"(tid));
+      // nout <= m / 2 = (E / 2) nt, since N1 <= nf1 / 2
+#pragma unroll
+      for (int e = 0; e < E / 2; ++e) {
+        const int i = tid + e * nt;
+        if (i < nout) {
+          const int col = S * i + c0;
+          const int k1 = col >= h1 ? col - h1 : nf1 + col - h1;
+          hb[h_slot(g, hs + i)] = lds[fftc::pad(k1 / S)];
+        }
+      }
+      __syncthreads();  // every output read: clear the block for the next plane
+      for (int i = tid; i < ntot; i += nt) lds[i] = fftc::mk((T)0, (T)0);
+      __syncthreads();  // cleared before the next plane's spread
     }
-    __syncthreads();  // every output read: clear the block for the next plane
-    for (int i = tid; i < ntot; i += nt) lds[i] = fftc::mk((T)0, (T)0);
-    __syncthreads();  // cleared before the next plane's spread
   }
 }
 
@@ -745,8 +826,13 @@ __global__ void __launch_bounds__(256) row_ifft_interp_part_kernel(
   }
   const auto tv = lds_twiddles<T>(lds + fftc::padded_size(m), tw, nf1);
   if (threadIdx.x < 64) walk_table_build(wt, off, g, p, row0, 1);
+  // FP32 keeps the inputs in registers for the first stage; FP64 parks them
+  // in LDS (held across the record loads they would cost 64 registers)
+  constexpr bool regs = sizeof(T) == 4;
+  if (!regs) {
 #pragma unroll
-  for (int e = 0; e < E; ++e) lds[fftc::pad(threadIdx.x + e * nt)] = v[e];
+    for (int e = 0; e < E; ++e) lds[fftc::pad(threadIdx.x + e * nt)] = v[e];
+  }
   __syncthreads();
   const int total = wt.total;
   if (total == 0) return;
@@ -758,7 +844,10 @@ __global__ void __launch_bounds__(256) row_ifft_interp_part_kernel(
     r0 = recs[idx0];
     load_taps<T, KWM>(taps, idx0, q0);
   }
-  lds_fft_part<T, E, 16, S>(lds, lgm, sign, tv);
+  if constexpr (regs)
+    lds_fft_part_from_regs<T, E, 16, S>(lds, v, lgm, sign, tv);
+  else
+    lds_fft_part<T, E, 16, S>(lds, lgm, sign, tv);
   cpx<T> wh[KWM];
 #pragma unroll
   for (int i = 0; i < KWM; ++i) {

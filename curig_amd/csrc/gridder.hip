@@ -280,6 +280,13 @@ int check_args(const Problem &pb) {
 
 }  // namespace
 
+// FP32 columns of 16384 run the register-resident column kernels as FP64 does
+// (1: yes; 0: the LDS column kernels)
+#ifndef CURIG_COLREG_FP32
+#define CURIG_COLREG_FP32 1
+#endif
+inline bool col_reg_fp32() { return CURIG_COLREG_FP32 != 0; }
+
 int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, double eps, double sigma, int sign,
                   double wmin_m, double wmax_m, bool dbl, const Tuning *tune, Geom *gp, KernelOpts *kop,
                   const GridSpec *spec) {
@@ -388,18 +395,18 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
     const bool row_whole = rb + 2048 <= lds_max && rpw * g.nf1 / 16 <= max_threads;
     const bool row_split = rpw == 1 && g.nf1 >= 4096 && g.nf1 / kRowSplit <= 16 * 256 && rbp + 2048 <= lds_max / 2;
     const bool col_whole = cb <= lds_max && cpw * g.nf2 / 16 <= max_threads;
-    // FP64 columns of 16384: held in registers, exchanged through LDS one
-    // component at a time
-    const size_t cbr = colreg_lds_bytes();
-    const bool col_reg = dbl && g.nf2 == 16384 && cbr <= lds_max;
+    // columns of 16384 held in registers (FP64: exchanged through LDS one
+    // component at a time; FP32: the first and last stages in registers)
+    const size_t cbr = colreg_lds_bytes(cs);
+    const bool col_reg = g.nf2 == 16384 && cbr <= lds_max && (dbl || col_reg_fp32());
     if ((row_whole || row_split) && (col_whole || col_reg) && rpw * g.nf1 >= 16 * 64 && cpw * g.nf2 >= 16) {
       g.fft_lds = 1;
       g.rpw = rpw;
-      g.cpw = col_whole ? cpw : 1;
-      g.colreg = col_whole ? 0 : 1;
+      g.colreg = col_reg ? 1 : 0;
+      g.cpw = col_reg ? 1 : cpw;
       g.nslot = g.N1;
       g.lds_row_bytes = row_whole ? (int)rb : 0;
-      g.lds_col_bytes = (int)(col_whole ? cb : cbr);
+      g.lds_col_bytes = (int)(col_reg ? cbr : cb);
       // rows too long for two workgroups per CU run split kRowSplit ways
       if (!row_whole || (rpw == 1 && rb + 2048 > lds_max / 2 && g.nf1 >= 4096)) {
         g.rsplit = kRowSplit;

@@ -1040,13 +1040,27 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) col_inv_kernel(const cpx
 // --------------------------------------------------------------------------
 namespace colr {
 constexpr int N = 16384, NT = 1024;
-// doubles of the exchange buffer (rounded up to 16 bytes: the twiddles follow)
+// elements of the exchange buffer (rounded up to 16 bytes: the twiddles
+// follow): kBuf doubles in FP64, kBuf complex values in FP32
 constexpr int kBuf = fftc::padded_size(N) + 1;
+template <typename T> __device__ __forceinline__ constexpr int buf_scalars() { return sizeof(T) == 4 ? 2 * kBuf : kBuf; }
 
-// v through LDS, one component at a time: stored at padded position st(r),
-// reloaded from ld(r) (each a base plus a constant step, one address register)
+// v through LDS: stored at padded position st(r), reloaded from ld(r) (each a
+// base plus a constant step, one address register); in FP32 as complex values,
+// in FP64 one component at a time (the column of 16384 complex doubles does
+// not fit the LDS)
 template <typename T, typename St, typename Ld>
 __device__ __forceinline__ void exchange(cpx<T> *v, T *buf, St st, Ld ld) {
+  if constexpr (sizeof(T) == 4) {
+    cpx<T> *b = reinterpret_cast<cpx<T> *>(buf);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) b[st(r)] = v[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = b[ld(r)];
+    __syncthreads();
+    return;
+  }
 #pragma unroll
   for (int r = 0; r < 16; ++r) buf[st(r)] = v[r].x;
   __syncthreads();
@@ -1154,14 +1168,19 @@ __global__ void __launch_bounds__(colr::NT) col_fwd_reg_kernel(const cpx<T> *__r
   const int sl = xcd_group(blockIdx.x, gridDim.x);  // column slot
   const int col = slot_column(sl, g);
   if (col < 0) return;
-  const auto tv = lds_twiddles<T>(reinterpret_cast<cpx<T> *>(buf + colr::kBuf), tw, N);
+  const auto tv = lds_twiddles<T>(reinterpret_cast<cpx<T> *>(buf + colr::buf_scalars<T>()), tw, N);
   // output q of this thread: X[j + 1024 (q mod 4) + 4096 r], r = 0 (q < 4) or 3;
   // its image row, or -1 (rows and w factors are recomputed per plane: held
   // across the plane loop they would not fit the registers beside the column)
   auto out_row = [&](int q) { return image_index(colr::opaque_tid() + 1024 * (q & 3) + (q < 4 ? 0 : 3 * 4096), N2, N); };
-  double ar[8], ai[8];
+  double ar[8], ai[8], zq[8];
 #pragma unroll
-  for (int q = 0; q < 8; ++q) ar[q] = ai[q] = 0.0;
+  for (int q = 0; q < 8; ++q) {
+    ar[q] = ai[q] = 0.0;
+    // FP32 columns leave room to keep the w arguments across the planes
+    const int row = out_row(q);
+    zq[q] = (sizeof(T) == 4 && row >= 0) ? pixel_zp(row, col, g) : 0.0;
+  }
   __syncthreads();
   const cpx<T> *hs = H + h_slot(g, sl);
   for (int k = 0; k < nb; ++k) {
@@ -1177,7 +1196,7 @@ __global__ void __launch_bounds__(colr::NT) col_fwd_reg_kernel(const cpx<T> *__r
       if (row >= 0) {
         const cpx<T> f = v[4 * (q & 3) + (q < 4 ? 0 : 3)];
         double sn, cs;
-        plane_phase<T>(pixel_zp(row, col, g) * m, &sn, &cs);
+        plane_phase<T>((sizeof(T) == 4 ? zq[q] : pixel_zp(row, col, g)) * m, &sn, &cs);
         const double fr = (double)f.x, fi = (double)f.y;
         ar[q] += fr * cs - fi * sn;
         ai[q] += fr * sn + fi * cs;
@@ -1217,12 +1236,23 @@ __global__ void __launch_bounds__(colr::NT) col_inv_reg_kernel(const cpx<T> *__r
   const int sl = xcd_group(blockIdx.x, gridDim.x);  // column slot
   const int col = slot_column(sl, g);
   if (col < 0) return;
-  const auto tv = lds_twiddles<T>(reinterpret_cast<cpx<T> *>(buf + colr::kBuf), tw, N);
+  const auto tv = lds_twiddles<T>(reinterpret_cast<cpx<T> *>(buf + colr::buf_scalars<T>()), tw, N);
   // input q of this thread: x[j + 1024 r], r = q (q < 4) or q + 8; its image
   // row, or -1 (recomputed per plane, as in col_fwd_reg_kernel)
   auto in_row = [&](int q) { return image_index(colr::opaque_tid() + 1024 * (q < 4 ? q : q + 8), N2, N); };
-  __syncthreads();
+  // FP32 columns leave room to keep the w arguments and the image values
+  // across the planes
+  constexpr bool keep = sizeof(T) == 4;
+  double zq[8];
+  cpx<T> gq[8];
   const cpx<T> *gc = gT + (size_t)col * N2;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int row = in_row(q);
+    zq[q] = (keep && row >= 0) ? pixel_zp(row, col, g) : 0.0;
+    gq[q] = (keep && row >= 0) ? gc[row] : fftc::mk((T)0, (T)0);
+  }
+  __syncthreads();
   cpx<T> *hs = H + h_slot(g, sl);
   for (int k = 0; k < nb; ++k) {
     const double m = (double)(p0 + k - g.P / 2);
@@ -1233,9 +1263,9 @@ __global__ void __launch_bounds__(colr::NT) col_inv_reg_kernel(const cpx<T> *__r
     for (int q = 0; q < 8; ++q) {
       const int row = in_row(q);
       if (row >= 0) {
-        const cpx<T> gv = gc[row];
+        const cpx<T> gv = keep ? gq[q] : gc[row];
         double sn, cs;
-        plane_phase<T>(-pixel_zp(row, col, g) * m, &sn, &cs);
+        plane_phase<T>(-(keep ? zq[q] : pixel_zp(row, col, g)) * m, &sn, &cs);
         const double gr = (double)gv.x, gi = (double)gv.y;
         v[q < 4 ? q : q + 8] = fftc::mk((T)(gr * cs - gi * sn), (T)(gr * sn + gi * cs));
       }
@@ -1327,11 +1357,9 @@ int allow_row_interp(const Geom &g) {
 template <typename T>
 void lds_cols_forward(const Geom &g, int p0, int nb, const cpx<T> *H, double2 *accT, const cpx<T> *tw2, int first,
                       hipStream_t st) {
-  if constexpr (sizeof(T) == 8) {
-    if (g.colreg) {
-      col_fwd_reg_kernel<T><<<col_groups(g), colr::NT, g.lds_col_bytes, st>>>(H, accT, tw2, g, p0, nb, first);
-      return;
-    }
+  if (g.colreg) {
+    col_fwd_reg_kernel<T><<<col_groups(g), colr::NT, g.lds_col_bytes, st>>>(H, accT, tw2, g, p0, nb, first);
+    return;
   }
   col_fwd_kernel<T, 16><<<col_groups(g), g.cpw * g.nf2 / 16, g.lds_col_bytes, st>>>(H, accT, tw2, g, p0, nb, first);
 }
@@ -1340,23 +1368,19 @@ void lds_cols_forward(const Geom &g, int p0, int nb, const cpx<T> *H, double2 *a
 template <typename T>
 void lds_cols_backward(const Geom &g, int p0, int nb, const cpx<T> *gT, cpx<T> *H, const cpx<T> *tw2,
                        hipStream_t st) {
-  if constexpr (sizeof(T) == 8) {
-    if (g.colreg) {
-      col_inv_reg_kernel<T><<<col_groups(g), colr::NT, g.lds_col_bytes, st>>>(gT, H, tw2, g, p0, nb);
-      return;
-    }
+  if (g.colreg) {
+    col_inv_reg_kernel<T><<<col_groups(g), colr::NT, g.lds_col_bytes, st>>>(gT, H, tw2, g, p0, nb);
+    return;
   }
   col_inv_kernel<T, 16><<<col_groups(g), g.cpw * g.nf2 / 16, g.lds_col_bytes, st>>>(gT, H, tw2, g, p0, nb);
 }
 
 template <typename T>
 int allow_cols(const Geom &g) {
-  if constexpr (sizeof(T) == 8) {
-    if (g.colreg) {
-      CURIG_TRY(allow_lds(col_fwd_reg_kernel<T>, g.lds_col_bytes));
-      CURIG_TRY(allow_lds(col_inv_reg_kernel<T>, g.lds_col_bytes));
-      return kOk;
-    }
+  if (g.colreg) {
+    CURIG_TRY(allow_lds(col_fwd_reg_kernel<T>, g.lds_col_bytes));
+    CURIG_TRY(allow_lds(col_inv_reg_kernel<T>, g.lds_col_bytes));
+    return kOk;
   }
   CURIG_TRY(allow_lds(col_fwd_kernel<T, 16>, g.lds_col_bytes));
   CURIG_TRY(allow_lds(col_inv_kernel<T, 16>, g.lds_col_bytes));

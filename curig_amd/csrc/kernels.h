@@ -59,11 +59,11 @@ struct Bufs {
 // to the row kernels' unrolled width (8 or 16).
 inline int tap_width(int kw) { return kw <= 8 ? 8 : 16; }
 
-// Dynamic LDS of the FP64 column kernels for columns of 16384 (held in
-// registers): the one-component exchange buffer, padded and rounded up to 16
-// bytes, then the column twiddles.
-inline size_t colreg_lds_bytes() {
-  return (size_t)(fftc::padded_size(16384) + 1) * 8 + (size_t)fftc::twiddle_entries(16384) * 16;
+// Dynamic LDS of the column kernels for columns of 16384 held in registers:
+// the exchange buffer (FP64: one component; FP32: complex values), padded and
+// rounded up to 16 bytes, then the column twiddles (cs bytes per complex).
+inline size_t colreg_lds_bytes(size_t cs) {
+  return (size_t)(fftc::padded_size(16384) + 1) * 8 + (size_t)fftc::twiddle_entries(16384) * cs;
 }
 
 inline int nblocks(long n, int bs = 256, long cap = 1 << 20) {

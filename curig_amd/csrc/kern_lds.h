@@ -246,37 +246,89 @@ __device__ __forceinline__ void lds_fft_part_from_regs(cpx<T> *buf, cpx<T> *v, i
 // A 4096-point part (256 threads, three radix-16 stages) whose last stage ends
 // in registers: thread j returns X[j + 256 r] in v[r], having cleared the LDS
 // elements it read last (every element of the part, for the next spread).
-template <typename T, int S, typename Tw>
-__device__ __forceinline__ void fft4096_part_to_regs(cpx<T> *buf, cpx<T> *v, int sign, const Tw &tw) {
-  constexpr int N = 4096, NT = 256;
-  const fftc::Layout L{N, 1, N, 1, false};
+// Thread j's stage twiddles are the same for every plane: tw2 and tw3 hold
+// w, w^2, w^4, w^8 of stages 2 (w = W256^(j mod 16)) and 3 (w = W4096^j),
+// sign applied (part_twiddles); the other powers are products of these.
+template <typename T> struct PartTw {
+  cpx<T> tw2[4], tw3[4];
+};
+
+template <typename T, typename Tw>
+__device__ __forceinline__ void part_twiddles(PartTw<T> &pt, const Tw &tw, int nf1, int sign) {
+  const int j = threadIdx.x, n = nf1 - 1, st = nf1 / 4096;  // the table has nf1 = S 4096 entries
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const cpx<T> a = tw[((((j & 15) * 16) << i) * st) & n];
+    const cpx<T> b = tw[((j << i) * st) & n];
+    pt.tw2[i] = fftc::mk(a.x, (T)sign * a.y);
+    pt.tw3[i] = fftc::mk(b.x, (T)sign * b.y);
+  }
+}
+
+// u[r] *= w^r for r = 1..15 from p = (w, w^2, w^4, w^8); the powers pass
+// through an empty asm so the eleven products stay at their use (hoisted out
+// of a plane loop they would hold 22 more registers)
+template <typename T> __device__ __forceinline__ void apply_tw16(cpx<T> *u, const cpx<T> *pin) {
+  cpx<T> p[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    p[i] = pin[i];
+    __asm__ volatile("" : "+v"(p[i].x), "+v"(p[i].y));
+  }
+#pragma unroll
+  for (int r = 1; r < 16; ++r) {
+    const int lo = __builtin_ctz(r);
+    cpx<T> w = p[lo];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (((r >> i) & 1) && i != lo) w = fftc::mul(w, p[i]);
+    u[r] = fftc::mul(u[r], w);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void fft4096_part_to_regs(cpx<T> *buf, cpx<T> *v, int sign, const PartTw<T> &pt) {
+  // every stage's butterfly j reads x[j + 256 r] at pad(j) + 272 r
   int tid = threadIdx.x;
   __asm__ volatile("" : "+v"(tid));
-  fftc::stage_load<T, 16, 16>(buf, v, L, 1, sign, tw, N * S, tid, NT);
+  {
+    const cpx<T> *b = buf + fftc::pad(tid);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = b[272 * r];
+  }
+  fftc::dft_small<T, 16>(v, sign);
   __syncthreads();
-  fftc::stage_store<T, 16, 16>(buf, v, L, 1, tid, NT);
+  {
+    cpx<T> *b = buf + 17 * tid;  // stage 1 outputs: pad(16 j + r) = 17 j + r
+#pragma unroll
+    for (int r = 0; r < 16; ++r) b[r] = v[r];
+  }
   __syncthreads();
   int tid2 = threadIdx.x;
   __asm__ volatile("" : "+v"(tid2));
-  fftc::stage_load<T, 16, 16>(buf, v, L, 16, sign, tw, N * S, tid2, NT);
+  {
+    const cpx<T> *b = buf + fftc::pad(tid2);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = b[272 * r];
+  }
+  apply_tw16(v, pt.tw2);
+  fftc::dft_small<T, 16>(v, sign);
   __syncthreads();
-  fftc::stage_store<T, 16, 16>(buf, v, L, 16, tid2, NT);
+  {
+    const int k = tid2 & 15;
+    cpx<T> *b = buf + fftc::pad(16 * (tid2 - k) + k);  // stage 2 outputs: pad(d) + 17 r
+#pragma unroll
+    for (int r = 0; r < 16; ++r) b[17 * r] = v[r];
+  }
   __syncthreads();
   int tid3 = threadIdx.x;
   __asm__ volatile("" : "+v"(tid3));
-  // last stage: butterfly j reads x[j + 256 r] (pad(j) + 272 r) and its
-  // outputs are X[j + 256 r]
   cpx<T> *own = buf + fftc::pad(tid3);
 #pragma unroll
   for (int r = 0; r < 16; ++r) v[r] = own[272 * r];
 #pragma unroll
   for (int r = 0; r < 16; ++r) own[272 * r] = fftc::mk((T)0, (T)0);
-  {
-    cpx<T> w[16];
-    fftc::twiddle_powers<T, 16>(w, tw, tid3 * (N * S / (256 * 16)));
-#pragma unroll
-    for (int r = 1; r < 16; ++r) v[r] = fftc::mulw(v[r], w[r].x, (T)sign * w[r].y);
-  }
+  apply_tw16(v, pt.tw3);
   fftc::dft_small<T, 16>(v, sign);
 }
 
@@ -672,13 +724,15 @@ __global__ void __launch_bounds__(256) spread_row_part_planes_kernel(
   // this part's image columns: grid_index(col) = col - N1/2 (mod nf1) = h (mod S)
   const int N1 = g.N1, h1 = N1 / 2, c0 = (h + h1) % S, nout = (N1 - c0 + S - 1) / S;
   const int hs = h << g.slot_lg;  // this part's first column slot
-  // this thread's first record of plane p0
+  // the stage twiddles of this thread (FP32 parts of 4096: kept across planes)
+  PartTw<T> pt;
+  if (sizeof(T) == 4 && E == 16 && m == 4096) part_twiddles(pt, tv, nf1, sign);
+  // this thread's first record of plane p0 (its taps are loaded when spread)
   Rec<T> r0;
-  Taps<T, KWM> q0;
+  int idx0 = 0;
   if (j < wt[0].total) {
-    const int idx = walk_record(wt[0], j);
-    r0 = recs[idx];
-    load_taps<T, KWM>(taps, idx, q0);
+    idx0 = walk_record(wt[0], j);
+    r0 = recs[idx0];
   }
   int cur = 0;
   for (int pl = 0; pl < nb; ++pl, cur ^= 1) {
@@ -686,7 +740,11 @@ __global__ void __launch_bounds__(256) spread_row_part_planes_kernel(
     const bool more = pl + 1 < nb;
     WalkAhead wa;
     if (j < 64 && more) walk_issue(wa, off, g, p + 1, row0);
-    if (j < total) spread_row_part<T, KWM>(r0, q0, ldsf, tv, whs, g, p, row0, h, m - 1, sign);
+    if (j < total) {
+      Taps<T, KWM> q0;
+      load_taps<T, KWM>(taps, idx0, q0);
+      spread_row_part<T, KWM>(r0, q0, ldsf, tv, whs, g, p, row0, h, m - 1, sign);
+    }
     for (int t = j + nt; t < total; t += nt) {
       const int ia = walk_record(wt[cur], t);
       Taps<T, KWM> ta;
@@ -697,9 +755,8 @@ __global__ void __launch_bounds__(256) spread_row_part_planes_kernel(
     __syncthreads();  // spread done; the next plane's walk table is complete
     // the next plane's first record, in flight during this plane's transform
     if (more && j < wt[cur ^ 1].total) {
-      const int idx = walk_record(wt[cur ^ 1], j);
-      r0 = recs[idx];
-      load_taps<T, KWM>(taps, idx, q0);
+      idx0 = walk_record(wt[cur ^ 1], j);
+      r0 = recs[idx0];
     }
     cpx<T> *hb = H + h_plane(g, pl) + row0 * kTile;
     if (total == 0) {
@@ -714,7 +771,7 @@ __global__ void __launch_bounds__(256) spread_row_part_planes_kernel(
       // image column for r < 4 and r >= 12 only; its slot steps by 64 tiles
       // per r (as in row_ifft_interp_part_kernel's inputs)
       cpx<T> v[16];
-      fft4096_part_to_regs<T, S>(lds, v, sign, tv);
+      fft4096_part_to_regs<T>(lds, v, sign, pt);
       const int tid = threadIdx.x, q = (h + h1) / S, k0 = S * tid + h + h1;
       cpx<T> *a0 = hb + h_slot(g, hs + tid + q);
       const long estep = (long)(256 >> kTileLog2) * g.tstride, mstep = (long)(4096 >> kTileLog2) * g.tstride;

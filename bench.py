@@ -788,8 +788,10 @@ def main():
     spread = ("spread_row_part_planes_kernel" if split else "spread_row_fft_kernel") if lds else "spread_tile_kernel"
     kernels = {spread: (kb["grid"], mean("t_grid", "m2d"))}
     if lds:
-        kernels["col_fwd_kernel"] = (kb["col_fwd"], mean("t_fft", "m2d"))
-        kernels["col_inv_kernel"] = (kb["col_inv"], mean("t_fft", "d2m"))
+        # columns of 16384 run the register-resident column kernels
+        reg = "_reg" if stats_by_ch[0]["nf2"] == 16384 else ""
+        kernels[f"col_fwd{reg}_kernel"] = (kb["col_fwd"], mean("t_fft", "m2d"))
+        kernels[f"col_inv{reg}_kernel"] = (kb["col_inv"], mean("t_fft", "d2m"))
         kernels[f"row_ifft_interp{part}_kernel"] = (kb["row_interp"], mean("t_grid", "d2m"))
     per_kernel = {k: {"bytes_per_launch": round(b / launches), "avg_launch_ms": round(t / launches * 1e3, 3),
                       "achieved_GBps": round(b / t / 1e9, 1), "frac": round(b / t / 1e9 / HBM_PEAK_GBS, 4)}

@@ -286,7 +286,7 @@ template <typename T> __device__ __forceinline__ void apply_tw16(cpx<T> *u, cons
   }
 }
 
-template <typename T>
+template <typename T, bool CLEAR = true>
 __device__ __forceinline__ void fft4096_part_to_regs(cpx<T> *buf, cpx<T> *v, int sign, const PartTw<T> &pt) {
   // every stage's butterfly j reads x[j + 256 r] at pad(j) + 272 r
   int tid = threadIdx.x;
@@ -326,8 +326,10 @@ __device__ __forceinline__ void fft4096_part_to_regs(cpx<T> *buf, cpx<T> *v, int
   cpx<T> *own = buf + fftc::pad(tid3);
 #pragma unroll
   for (int r = 0; r < 16; ++r) v[r] = own[272 * r];
+  if (CLEAR) {
 #pragma unroll
-  for (int r = 0; r < 16; ++r) own[272 * r] = fftc::mk((T)0, (T)0);
+    for (int r = 0; r < 16; ++r) own[272 * r] = fftc::mk((T)0, (T)0);
+  }
   apply_tw16(v, pt.tw3);
   fftc::dft_small<T, 16>(v, sign);
 }
@@ -558,6 +560,26 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) spread_row_fft_kernel(co
     spread_rows<T, KWM>(ra, ta, ldsf, g, p, row0);
   }
   __syncthreads();
+  if (sizeof(T) == 4 && E == 16 && rpw == 1 && nf1 == 4096) {
+    // one row of 4096 (256 threads): the last stage ends in registers and
+    // writes the image columns from there.  X[j + 256 r] is image column
+    // j + 256 r + N1 / 2 (r < 4) or that minus nf1 (r >= 12); the others are
+    // outside the image (N1 <= nf1 / 2).
+    PartTw<T> pt;
+    part_twiddles(pt, tv, nf1, +1);
+    cpx<T> v[16];
+    fft4096_part_to_regs<T, false>(lds, v, +1, pt);
+    const int k0 = threadIdx.x + h1;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      if (r < 4) {
+        if (k0 + 256 * r < N1) hb[h_slot(g, k0 + 256 * r)] = v[r];
+      } else if (r >= 12) {
+        if (k0 + 256 * r >= nf1) hb[h_slot(g, k0 + 256 * r - nf1)] = v[r];
+      }
+    }
+    return;
+  }
   lds_fft_rows<T, E, 16>(lds, 31 - __clz(nf1), rpw, +1, tv);
   // pruned write: all LDS reads first, then the stores (kTile-column chunks);
   // offsets from the row's base in 32 bits (a plane holds < 2^31 elements)
@@ -616,8 +638,13 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) row_ifft_interp_kernel(c
   }
   const auto tv = lds_twiddles<T>(lds + fftc::padded_size(rpw * nf1), tw, nf1);
   if (threadIdx.x < 64) walk_table_build(wt, off, g, p, row0, rpw);
+  // FP32 rows of 4096 (256 threads): v is the first radix-16 stage's
+  // butterfly, so it runs in registers before the first LDS store
+  const bool regs = sizeof(T) == 4 && KWM <= 8 && E == 16 && rpw == 1 && nf1 == 4096;
+  if (!regs) {
 #pragma unroll
-  for (int e = 0; e < E; ++e) lds[fftc::pad(threadIdx.x + e * nt)] = v[e];
+    for (int e = 0; e < E; ++e) lds[fftc::pad(threadIdx.x + e * nt)] = v[e];
+  }
   __syncthreads();
   const int total = wt.total;
   if (total == 0) return;  // no visibility reaches these rows of this plane
@@ -630,7 +657,10 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) row_ifft_interp_kernel(c
     r0 = recs[idx0];
     load_taps<T, KWM>(taps, idx0, q0);
   }
-  lds_fft_rows<T, E, 16>(lds, lg, rpw, sign, tv);
+  if (regs)
+    part_stages_from_regs<T, E, 16, 4096, 1>(lds, v, sign, tv);
+  else
+    lds_fft_rows<T, E, 16>(lds, lg, rpw, sign, tv);
   if (threadIdx.x < total) interp_rows<T, KWM>(r0, q0, idx0, lds, g, p, row0, vacc);
   for (int t = threadIdx.x + nt; t < total; t += nt) {
     const int idx = walk_record(wt, t);

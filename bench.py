@@ -785,7 +785,9 @@ def main():
     lds = bool(stats_by_ch[0]["fft_lds"])
     split = stats_by_ch[0]["rsplit"] > 0  # rows split over several workgroups
     part = "_part" if split else ""
-    spread = ("spread_row_part_planes_kernel" if split else "spread_row_fft_kernel") if lds else "spread_tile_kernel"
+    # FP32 rows of 4096 run the plane-loop row kernel whole (one part)
+    planes = split or (run.prec == 1 and stats_by_ch[0]["nf1"] == 4096)
+    spread = ("spread_row_part_planes_kernel" if planes else "spread_row_fft_kernel") if lds else "spread_tile_kernel"
     kernels = {spread: (kb["grid"], mean("t_grid", "m2d"))}
     if lds:
         # columns of 16384 run the register-resident column kernels

@@ -286,12 +286,23 @@ template <typename T> __device__ __forceinline__ void apply_tw16(cpx<T> *u, cons
   }
 }
 
-template <typename T, bool CLEAR = true>
+template <typename T, bool ACC = false>
 __device__ __forceinline__ void fft4096_part_to_regs(cpx<T> *buf, cpx<T> *v, int sign, const PartTw<T> &pt) {
   // every stage's butterfly j reads x[j + 256 r] at pad(j) + 272 r
   int tid = threadIdx.x;
   __asm__ volatile("" : "+v"(tid));
-  {
+  if (ACC) {
+    // the input is the FP64 accumulator (cell c at double2 c), cleared as it
+    // is read; the FP32 stages below then use its first 4352 * 8 bytes
+    double2 *a = reinterpret_cast<double2 *>(buf) + tid;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const double2 x = a[256 * r];
+      v[r] = fftc::mk((T)x.x, (T)x.y);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) a[256 * r] = make_double2(0.0, 0.0);
+  } else {
     const cpx<T> *b = buf + fftc::pad(tid);
 #pragma unroll
     for (int r = 0; r < 16; ++r) v[r] = b[272 * r];
@@ -326,10 +337,8 @@ __device__ __forceinline__ void fft4096_part_to_regs(cpx<T> *buf, cpx<T> *v, int
   cpx<T> *own = buf + fftc::pad(tid3);
 #pragma unroll
   for (int r = 0; r < 16; ++r) v[r] = own[272 * r];
-  if (CLEAR) {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) own[272 * r] = fftc::mk((T)0, (T)0);
-  }
+  for (int r = 0; r < 16; ++r) own[272 * r] = fftc::mk((T)0, (T)0);
   apply_tw16(v, pt.tw3);
   fftc::dft_small<T, 16>(v, sign);
 }
@@ -560,26 +569,6 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) spread_row_fft_kernel(co
     spread_rows<T, KWM>(ra, ta, ldsf, g, p, row0);
   }
   __syncthreads();
-  if (sizeof(T) == 4 && E == 16 && rpw == 1 && nf1 == 4096) {
-    // one row of 4096 (256 threads): the last stage ends in registers and
-    // writes the image columns from there.  X[j + 256 r] is image column
-    // j + 256 r + N1 / 2 (r < 4) or that minus nf1 (r >= 12); the others are
-    // outside the image (N1 <= nf1 / 2).
-    PartTw<T> pt;
-    part_twiddles(pt, tv, nf1, +1);
-    cpx<T> v[16];
-    fft4096_part_to_regs<T, false>(lds, v, +1, pt);
-    const int k0 = threadIdx.x + h1;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      if (r < 4) {
-        if (k0 + 256 * r < N1) hb[h_slot(g, k0 + 256 * r)] = v[r];
-      } else if (r >= 12) {
-        if (k0 + 256 * r >= nf1) hb[h_slot(g, k0 + 256 * r - nf1)] = v[r];
-      }
-    }
-    return;
-  }
   lds_fft_rows<T, E, 16>(lds, 31 - __clz(nf1), rpw, +1, tv);
   // pruned write: all LDS reads first, then the stores (kTile-column chunks);
   // offsets from the row's base in 32 bits (a plane holds < 2^31 elements)
@@ -674,10 +663,27 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) row_ifft_interp_kernel(c
 // holds z_h[n mod m] = sum over the row's cells n of x[n] w^(sign h n), whose
 // m-point FFT gives the row transform at the indices S k + h (radix-S
 // decimation in frequency; w = e^{2 pi i / nf1}, m = nf1 / S).
+// The plane-loop row kernel's FP64 accumulation, for whole FP32 rows of 4096
+// (the register last stage): 4096 cells of 16 bytes, then the twiddle table.
+// Split rows keep FP32 accumulation: their workgroups walk fewer records per
+// item, and two workgroups per CU instead of four cost more than the atomics
+// save (16384^2 grids: 25.3 against 20.2 ms per launch; 4096^2: 1.72 against
+// 1.96 ms).
+constexpr int kAcc64Bytes = 4096 * 16;
+template <typename T, int E, int S> __host__ __device__ inline bool planes_acc64(int m) {
+  return sizeof(T) == 4 && E == 16 && S == 1 && m == 4096;
+}
+template <typename T, int S> inline int planes_lds_bytes(const Geom &g, int bytes) {
+  return planes_acc64<T, 16, S>(g.nf1 / S) ? kAcc64Bytes + fftc::twiddle_entries(g.nf1) * (int)sizeof(cpx<T>)
+                                           : bytes;
+}
+
+// acc: the block is an FP64 accumulator (cell c at doubles 2c, 2c + 1)
+// instead of the padded FP32 buffer.
 template <typename T, int KWM, bool PAD = true, typename Tw>
 __device__ __forceinline__ void spread_row_part(const Rec<T> rc, const Taps<T, KWM> &tp, T *ldsf, const Tw &tv,
                                                 const cpx<T> *wh, const Geom &g, int p, int row0, int h, int mm,
-                                                int sign) {
+                                                int sign, bool acc = false) {
   const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2;
   int k = p - rc.iw;
   if (k < 0) k += g.P;
@@ -705,9 +711,15 @@ __device__ __forceinline__ void spread_row_part(const Rec<T> rc, const Taps<T, K
         im = re * wh[i].y + im * wh[i].x;
         re = t;
       }
-      const int q = 2 * (PAD ? fftc::pad(cx & mm) : (cx & mm));
-      atomicAdd(ldsf + q, re);
-      atomicAdd(ldsf + q + 1, im);
+      if (acc) {
+        double *d = reinterpret_cast<double *>(ldsf) + 2 * (cx & mm);
+        atomicAdd(d, (double)re);
+        atomicAdd(d + 1, (double)im);
+      } else {
+        const int q = 2 * (PAD ? fftc::pad(cx & mm) : (cx & mm));
+        atomicAdd(ldsf + q, re);
+        atomicAdd(ldsf + q + 1, im);
+      }
     }
   }
 }
@@ -723,7 +735,9 @@ __device__ __forceinline__ void spread_row_part(const Rec<T> rc, const Taps<T, K
 // transformed, the walk table of plane p + 1 is complete and each thread's
 // first record of it is in flight, and the twiddle table and the part's
 // factors are loaded once (c3: 23.8 against 24.3 ms per 32-plane launch for
-// one workgroup per (row, part, plane)).
+// one workgroup per (row, part, plane)).  With S = 1 it runs whole FP32 rows
+// of 4096 (2048^2 images: 1.96 against 2.07 ms per launch for one workgroup
+// per (row, plane)).
 template <typename T, int E, int KWM, int S>
 __global__ void __launch_bounds__(256) spread_row_part_planes_kernel(
     const Rec<T> *__restrict__ recs, const T *__restrict__ taps, const int *__restrict__ off, cpx<T> *__restrict__ H,
@@ -738,7 +752,10 @@ __global__ void __launch_bounds__(256) spread_row_part_planes_kernel(
   // consecutive items of one XCD; the XCDs take such blocks in turn
   const int item = xcd_interleave(blockIdx.x, gridDim.x, 4 * S);
   const int row0 = item / S, h = item % S;
-  const int ntot = fftc::padded_size(m);
+  // FP32 parts of 4096 accumulate in FP64 (LDS float atomics are slow), in a
+  // block of 4096 x 16 bytes that the FP32 transform then reuses
+  const bool regs = sizeof(T) == 4 && E == 16 && m == 4096, acc = planes_acc64<T, E, S>(m);
+  const int ntot = acc ? kAcc64Bytes / (int)sizeof(cpx<T>) : fftc::padded_size(m);
   const auto tv = lds_twiddles<T>(lds + ntot, tw, nf1);
   if (j < 64) walk_table_build(wt[0], off, g, p0, row0, 1);
   for (int i = j; i < ntot; i += nt) lds[i] = fftc::mk((T)0, (T)0);
@@ -756,7 +773,7 @@ __global__ void __launch_bounds__(256) spread_row_part_planes_kernel(
   const int hs = h << g.slot_lg;  // this part's first column slot
   // the stage twiddles of this thread (FP32 parts of 4096: kept across planes)
   PartTw<T> pt;
-  if (sizeof(T) == 4 && E == 16 && m == 4096) part_twiddles(pt, tv, nf1, sign);
+  if (regs) part_twiddles(pt, tv, nf1, sign);
   // this thread's first record of plane p0 (its taps are loaded when spread)
   Rec<T> r0;
   int idx0 = 0;
@@ -773,13 +790,13 @@ __global__ void __launch_bounds__(256) spread_row_part_planes_kernel(
     if (j < total) {
       Taps<T, KWM> q0;
       load_taps<T, KWM>(taps, idx0, q0);
-      spread_row_part<T, KWM>(r0, q0, ldsf, tv, whs, g, p, row0, h, m - 1, sign);
+      spread_row_part<T, KWM>(r0, q0, ldsf, tv, whs, g, p, row0, h, m - 1, sign, acc);
     }
     for (int t = j + nt; t < total; t += nt) {
       const int ia = walk_record(wt[cur], t);
       Taps<T, KWM> ta;
       load_taps<T, KWM>(taps, ia, ta);
-      spread_row_part<T, KWM>(recs[ia], ta, ldsf, tv, whs, g, p, row0, h, m - 1, sign);
+      spread_row_part<T, KWM>(recs[ia], ta, ldsf, tv, whs, g, p, row0, h, m - 1, sign, acc);
     }
     if (j < 64 && more) walk_finish(wt[cur ^ 1], wa);
     __syncthreads();  // spread done; the next plane's walk table is complete
@@ -795,13 +812,16 @@ __global__ void __launch_bounds__(256) spread_row_part_planes_kernel(
       for (int i = j; i < nout; i += nt) hb[h_slot(g, hs + i)] = fftc::mk((T)0, (T)0);
       continue;
     }
-    if (sizeof(T) == 4 && E == 16 && m == 4096) {
+    if (regs) {
       // last stage in registers (FP32: FP64 would hold 32 more registers): output X[j + 256 r] (part index) is image
       // column S (j + 256 r) + h + N1 / 2 (folded by nf1), which can be an
       // image column for r < 4 and r >= 12 only; its slot steps by 64 tiles
       // per r (as in row_ifft_interp_part_kernel's inputs)
       cpx<T> v[16];
-      fft4096_part_to_regs<T>(lds, v, sign, pt);
+      if (acc)
+        fft4096_part_to_regs<T, true>(lds, v, sign, pt);
+      else
+        fft4096_part_to_regs<T, false>(lds, v, sign, pt);
       const int tid = threadIdx.x, q = (h + h1) / S, k0 = S * tid + h + h1;
       cpx<T> *a0 = hb + h_slot(g, hs + tid + q);
       const long estep = (long)(256 >> kTileLog2) * g.tstride, mstep = (long)(4096 >> kTileLog2) * g.tstride;
@@ -1377,16 +1397,30 @@ int allow_lds(K kernel, int bytes) {
 
 inline int col_groups(const Geom &g) { return (g.nslot + g.cpw - 1) / g.cpw; }
 
+template <typename T> inline bool whole_row_planes(const Geom &g) {
+  return sizeof(T) == 4 && g.rsplit == 0 && g.rpw == 1 && g.nf1 == 4096;
+}
+
 // ms2dirty, one batch on the LDS path: spread + row FFT -> H (column-tiled)
 template <typename T>
 void lds_spread_rows(const Rec<T> *recs, const T *taps, const int *off, const Geom &g, int p0, int nb, cpx<T> *H,
                      const cpx<T> *tw1, hipStream_t st) {
   if (g.rsplit == kRowSplit) {
-    const int nt = g.nf1 / (16 * kRowSplit), lb = g.lds_rowp_bytes, grd = kRowSplit * g.nf2;
+    const int nt = g.nf1 / (16 * kRowSplit), grd = kRowSplit * g.nf2;
+    const int lb = planes_lds_bytes<T, kRowSplit>(g, g.lds_rowp_bytes);
     if (g.kw <= 8)
       spread_row_part_planes_kernel<T, 16, 8, kRowSplit><<<grd, nt, lb, st>>>(recs, taps, off, H, tw1, g, p0, nb);
     else
       spread_row_part_planes_kernel<T, 16, 16, kRowSplit><<<grd, nt, lb, st>>>(recs, taps, off, H, tw1, g, p0, nb);
+    return;
+  }
+  if (whole_row_planes<T>(g)) {
+    // FP32 rows of 4096: one workgroup per row walks the batch's planes
+    const int lb = planes_lds_bytes<T, 1>(g, g.lds_row_bytes);
+    if (g.kw <= 8)
+      spread_row_part_planes_kernel<T, 16, 8, 1><<<g.nf2, 256, lb, st>>>(recs, taps, off, H, tw1, g, p0, nb);
+    else
+      spread_row_part_planes_kernel<T, 16, 16, 1><<<g.nf2, 256, lb, st>>>(recs, taps, off, H, tw1, g, p0, nb);
     return;
   }
   const dim3 grd(g.nf2 / g.rpw, nb);
@@ -1401,9 +1435,15 @@ template <typename T>
 int allow_spread_rows(const Geom &g) {
   CURIG_TRY(allow_lds(spread_row_fft_kernel<T, 16, 8>, g.lds_row_bytes));
   CURIG_TRY(allow_lds(spread_row_fft_kernel<T, 16, 16>, g.lds_row_bytes));
+  if (whole_row_planes<T>(g)) {
+    const int lb = planes_lds_bytes<T, 1>(g, g.lds_row_bytes);
+    CURIG_TRY((allow_lds(spread_row_part_planes_kernel<T, 16, 8, 1>, lb)));
+    CURIG_TRY((allow_lds(spread_row_part_planes_kernel<T, 16, 16, 1>, lb)));
+  }
   if (g.rsplit == kRowSplit) {
-    CURIG_TRY((allow_lds(spread_row_part_planes_kernel<T, 16, 8, kRowSplit>, g.lds_rowp_bytes)));
-    CURIG_TRY((allow_lds(spread_row_part_planes_kernel<T, 16, 16, kRowSplit>, g.lds_rowp_bytes)));
+    const int lb = planes_lds_bytes<T, kRowSplit>(g, g.lds_rowp_bytes);
+    CURIG_TRY((allow_lds(spread_row_part_planes_kernel<T, 16, 8, kRowSplit>, lb)));
+    CURIG_TRY((allow_lds(spread_row_part_planes_kernel<T, 16, 16, kRowSplit>, lb)));
   }
   return kOk;
 }

@@ -98,6 +98,8 @@ struct Geom {
   int fft_lds;
   int rpw;            // grid rows per workgroup in the row pass (rpw * nf1 = 16 * threads)
   int cpw;            // grid columns per workgroup in the column pass (cpw * nf2 = 16 * threads)
+  int cpw_inv;        // the same for the dirty2ms column pass (whole column tiles where they fit)
+  int lds_coli_bytes; // its LDS
   int nct;            // column tiles of a plane's row-transformed layout (nslot / kTile, rounded up)
   int tstride;        // elements from one column tile to the next: nf2 * kTile + a pad
   int nslot;          // column slots of that layout: N1, or nf1 / 2 with split rows

@@ -378,7 +378,7 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
   // workgroup's LDS, with the image at most half the grid on each axis
   g.fft_lds = 0;
   g.rpw = g.cpw = g.nct = g.tstride = g.nslot = g.slot_lg = g.lds_row_bytes = g.lds_col_bytes = g.rsplit =
-      g.lds_rowp_bytes = g.colreg = 0;
+      g.lds_rowp_bytes = g.colreg = g.cpw_inv = g.lds_coli_bytes = 0;
   auto pow2 = [](int n) { return n > 0 && (n & (n - 1)) == 0; };
   const size_t cs = dbl ? 16 : 8;
   const size_t lds_max = tn.lds_bytes > 0 ? (size_t)tn.lds_bytes : size_t(160) << 10;
@@ -407,6 +407,22 @@ int plan_geometry(int nrow_valid, int nx, int ny, double fov, double freq, doubl
       g.nslot = g.N1;
       g.lds_row_bytes = row_whole ? (int)rb : 0;
       g.lds_col_bytes = (int)(col_reg ? cbr : cb);
+      // the dirty2ms column pass writes H: FP32 workgroups of fewer than
+      // kTile columns leave each 32-byte chunk to several workgroups, which
+      // reach a plane at different times, and the L2 then writes the chunk's
+      // lines back more than once (2.4x the bytes at 4096^2); with whole tiles
+      // a workgroup writes whole chunks (1.0x)
+      g.cpw_inv = g.cpw;
+      g.lds_coli_bytes = g.lds_col_bytes;
+      for (int c = kTile; !dbl && !col_reg && cpw < c && c >= 2; c /= 2) {
+        // whole tiles, else half tiles (16-byte pieces of a chunk)
+        const size_t cbi = (size_t)(fftc::padded_size(c * g.nf2) + fftc::twiddle_entries(g.nf2)) * cs;
+        if (cbi <= lds_max && c * g.nf2 / 16 <= max_threads) {
+          g.cpw_inv = c;
+          g.lds_coli_bytes = (int)cbi;
+          break;
+        }
+      }
       // rows too long for two workgroups per CU run split kRowSplit ways
       if (!row_whole || (rpw == 1 && rb + 2048 > lds_max / 2 && g.nf1 >= 4096)) {
         g.rsplit = kRowSplit;
@@ -907,7 +923,7 @@ uint64_t graph_key(bool to_image, const Problem *pbs, int n, const std::vector<C
     k << cp.nvalid << g.N1 << g.N2 << g.nf1 << g.nf2 << g.P << g.kw << g.kww << g.hdeg << g.hcoef_d << g.hcoef_f
       << g.beta << g.c << g.dx << g.dy << g.divn << g.zout << g.C << g.D << g.gamma << g.h << g.uscale << g.vscale
       << g.wscale << g.T1 << g.T2 << g.ntx << g.nty << g.ntiles << g.Pb << g.nbatch << g.Pbin << g.nlayer
-      << g.nquad << g.fft_lds << g.rpw << g.cpw << g.nct << g.tstride << g.nslot << g.slot_lg << g.lds_row_bytes
+      << g.nquad << g.fft_lds << g.rpw << g.cpw << g.cpw_inv << g.lds_coli_bytes << g.nct << g.tstride << g.nslot << g.slot_lg << g.lds_row_bytes
       << g.lds_col_bytes << g.rsplit << g.lds_rowp_bytes << g.colreg;
   }
   k << b.dev << b.tabs << b.hist << b.off << b.bsum << b.perm << b.binrank << b.recs << b.taps << b.grid << b.H << b.acc

@@ -502,8 +502,8 @@ __device__ __forceinline__ int slot_column(int s, const Geom &g) {
 }
 
 // whether a column workgroup's slots [s0, s0 + cpw) hold any image column
-__device__ __forceinline__ bool slots_used(int s0, const Geom &g) {
-  for (int c = 0; c < g.cpw; ++c)
+__device__ __forceinline__ bool slots_used(int s0, const Geom &g, int cpw) {
+  for (int c = 0; c < cpw; ++c)
     if (slot_column(s0 + c, g) >= 0) return true;
   return false;
 }
@@ -993,7 +993,7 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) col_fwd_kernel(const cpx
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
   const int cpw = g.cpw, nf2 = g.nf2, N2 = g.N2, nt = blockDim.x;
   const int s0 = xcd_group(blockIdx.x, gridDim.x) * cpw;  // first column slot
-  if (!slots_used(s0, g)) return;
+  if (!slots_used(s0, g, cpw)) return;
   const int lg = 31 - __clz(nf2);
   const int nout = cpw * N2;
   double ar[8], ai[8], zp[8];
@@ -1076,11 +1076,13 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) col_inv_kernel(const cpx
                                                                       int p0, int nb) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
-  const int cpw = g.cpw, nf2 = g.nf2, N2 = g.N2, nt = blockDim.x;
+  const int cpw = g.cpw_inv, nf2 = g.nf2, N2 = g.N2, nt = blockDim.x;
   const int s0 = xcd_group(blockIdx.x, gridDim.x) * cpw;  // first column slot
-  if (!slots_used(s0, g)) return;
+  if (!slots_used(s0, g, cpw)) return;
   const int lg = 31 - __clz(nf2);
   const int nout = cpw * N2, nzero = cpw * (nf2 - N2), h2 = N2 / 2;
+  bool full = true;  // every column slot of the workgroup holds an image column
+  for (int c = 0; c < cpw; ++c) full = full && slot_column(s0 + c, g) >= 0;
   cpx<T> gv[8];
   double zp[8];
 #pragma unroll
@@ -1123,15 +1125,61 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) col_inv_kernel(const cpx
     lds_fft_rows<T, E, 16>(lds, lg, cpw, -1, tv);
     int tid2 = threadIdx.x;
     __asm__ volatile("" : "+v"(tid2));
-    cpx<T> v[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = lds[fftc::pad(tid2 + e * nt)];
     cpx<T> *hk = H + h_plane(g, k);
+    if (sizeof(T) == 4 && kTile == 4 && cpw == 2) {
+      // half tiles: thread k2 writes row k2's two columns as one 16-byte piece
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int i = tid2 + e * nt;
-      const int ci = i >> lg, k2 = i & (nf2 - 1), sl = s0 + ci;
-      if (slot_column(sl, g) >= 0) hk[h_slot(g, sl) + (k2 << kTileLog2)] = v[e];
+      for (int q = 0; q < E / 2; ++q) {
+        const int k2 = tid2 + q * nt;
+        const cpx<T> a = lds[fftc::pad(k2)], b = lds[fftc::pad(nf2 + k2)];
+        cpx<T> *d = hk + h_slot(g, s0) + (k2 << kTileLog2);
+        if (full) {
+          *reinterpret_cast<float4 *>(d) = make_float4(a.x, a.y, b.x, b.y);
+        } else {
+          if (slot_column(s0, g) >= 0) d[0] = a;
+          if (slot_column(s0 + 1, g) >= 0) d[1] = b;
+        }
+      }
+    } else if (sizeof(T) == 4 && kTile == 4 && cpw % kTile == 0) {
+      // whole column tiles: 16-byte pieces, piece r of tile t holding row
+      // r / 2, columns 2 (r & 1) and 2 (r & 1) + 1 of the tile, so that
+      // consecutive lanes write consecutive 16 bytes of H
+      const int tlg = lg + 1;  // 2 nf2 pieces per tile
+      float4 w[E / 2];
+#pragma unroll
+      for (int q = 0; q < E / 2; ++q) {
+        const int P = tid2 + q * nt, t = P >> tlg, r = P & ((1 << tlg) - 1), c = kTile * t + 2 * (r & 1);
+        const cpx<T> a = lds[fftc::pad(c * nf2 + (r >> 1))], b = lds[fftc::pad((c + 1) * nf2 + (r >> 1))];
+        w[q] = make_float4(a.x, a.y, b.x, b.y);
+      }
+      if (full && cpw == kTile) {
+        // one tile: its pieces are contiguous
+        float4 *d = reinterpret_cast<float4 *>(hk + h_slot(g, s0)) + tid2;
+#pragma unroll
+        for (int q = 0; q < E / 2; ++q) d[q * nt] = w[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < E / 2; ++q) {
+          const int P = tid2 + q * nt, t = P >> tlg, r = P & ((1 << tlg) - 1), c = kTile * t + 2 * (r & 1);
+          cpx<T> *d = hk + h_slot(g, s0 + kTile * t) + 2 * r;  // 16-byte aligned: tiles start on 32 bytes
+          if (full) {
+            *reinterpret_cast<float4 *>(d) = w[q];
+          } else {
+            if (slot_column(s0 + c, g) >= 0) d[0] = fftc::mk((T)w[q].x, (T)w[q].y);
+            if (slot_column(s0 + c + 1, g) >= 0) d[1] = fftc::mk((T)w[q].z, (T)w[q].w);
+          }
+        }
+      }
+    } else {
+      cpx<T> v[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = lds[fftc::pad(tid2 + e * nt)];
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int i = tid2 + e * nt;
+        const int ci = i >> lg, k2 = i & (nf2 - 1), sl = s0 + ci;
+        if (slot_column(sl, g) >= 0) hk[h_slot(g, sl) + (k2 << kTileLog2)] = v[e];
+      }
     }
   }
 }
@@ -1395,7 +1443,7 @@ int allow_lds(K kernel, int bytes) {
   return kOk;
 }
 
-inline int col_groups(const Geom &g) { return (g.nslot + g.cpw - 1) / g.cpw; }
+inline int col_groups(const Geom &g, int cpw) { return (g.nslot + cpw - 1) / cpw; }
 
 template <typename T> inline bool whole_row_planes(const Geom &g) {
   return sizeof(T) == 4 && g.rsplit == 0 && g.rpw == 1 && g.nf1 == 4096;
@@ -1485,10 +1533,10 @@ template <typename T>
 void lds_cols_forward(const Geom &g, int p0, int nb, const cpx<T> *H, double2 *accT, const cpx<T> *tw2, int first,
                       hipStream_t st) {
   if (g.colreg) {
-    col_fwd_reg_kernel<T><<<col_groups(g), colr::NT, g.lds_col_bytes, st>>>(H, accT, tw2, g, p0, nb, first);
+    col_fwd_reg_kernel<T><<<col_groups(g, 1), colr::NT, g.lds_col_bytes, st>>>(H, accT, tw2, g, p0, nb, first);
     return;
   }
-  col_fwd_kernel<T, 16><<<col_groups(g), g.cpw * g.nf2 / 16, g.lds_col_bytes, st>>>(H, accT, tw2, g, p0, nb, first);
+  col_fwd_kernel<T, 16><<<col_groups(g, g.cpw), g.cpw * g.nf2 / 16, g.lds_col_bytes, st>>>(H, accT, tw2, g, p0, nb, first);
 }
 
 // dirty2ms, one batch on the LDS path: w-fill + inverse column FFT -> H (column-tiled)
@@ -1496,21 +1544,27 @@ template <typename T>
 void lds_cols_backward(const Geom &g, int p0, int nb, const cpx<T> *gT, cpx<T> *H, const cpx<T> *tw2,
                        hipStream_t st) {
   if (g.colreg) {
-    col_inv_reg_kernel<T><<<col_groups(g), colr::NT, g.lds_col_bytes, st>>>(gT, H, tw2, g, p0, nb);
+    col_inv_reg_kernel<T><<<col_groups(g, 1), colr::NT, g.lds_col_bytes, st>>>(gT, H, tw2, g, p0, nb);
     return;
   }
-  col_inv_kernel<T, 16><<<col_groups(g), g.cpw * g.nf2 / 16, g.lds_col_bytes, st>>>(gT, H, tw2, g, p0, nb);
+  col_inv_kernel<T, 16><<<col_groups(g, g.cpw_inv), g.cpw_inv * g.nf2 / 16, g.lds_coli_bytes, st>>>(gT, H, tw2, g, p0, nb);
 }
 
 template <typename T>
 int allow_cols(const Geom &g) {
+  // every column workgroup needs its LDS block (a launch with less would
+  // address LDS it does not own)
+  if (g.lds_col_bytes <= 0 || (!g.colreg && g.lds_coli_bytes <= 0)) {
+    set_error("column pass without an LDS size (%d, %d)", g.lds_col_bytes, g.lds_coli_bytes);
+    return kErrArg;
+  }
   if (g.colreg) {
     CURIG_TRY(allow_lds(col_fwd_reg_kernel<T>, g.lds_col_bytes));
     CURIG_TRY(allow_lds(col_inv_reg_kernel<T>, g.lds_col_bytes));
     return kOk;
   }
   CURIG_TRY(allow_lds(col_fwd_kernel<T, 16>, g.lds_col_bytes));
-  CURIG_TRY(allow_lds(col_inv_kernel<T, 16>, g.lds_col_bytes));
+  CURIG_TRY(allow_lds(col_inv_kernel<T, 16>, g.lds_coli_bytes));
   return kOk;
 }
 

@@ -286,12 +286,28 @@ template <typename T> __device__ __forceinline__ void apply_tw16(cpx<T> *u, cons
   }
 }
 
-template <typename T, bool ACC = false>
-__device__ __forceinline__ void fft4096_part_to_regs(cpx<T> *buf, cpx<T> *v, int sign, const PartTw<T> &pt) {
+// Cells per block of the block-floating-point spread (kAccFix below) of a
+// 4096-cell row part, and the blocks.
+constexpr int kFixBlkLog2 = 5, kFixBlks = 4096 >> kFixBlkLog2;
+constexpr int kFixNaN = 0x40000000;  // exponent of a block with a non-finite bound
+
+template <typename T, int IN = 0>
+__device__ __forceinline__ void fft4096_part_to_regs(cpx<T> *buf, cpx<T> *v, int sign, const PartTw<T> &pt,
+                                                     const int *fexp = nullptr) {
   // every stage's butterfly j reads x[j + 256 r] at pad(j) + 272 r
   int tid = threadIdx.x;
   __asm__ volatile("" : "+v"(tid));
-  if (ACC) {
+  if (IN == 2) {
+    // block floating point (kAccFix): int32 pairs at the padded positions,
+    // cell j + 256 r in block (j + 256 r) >> kFixBlkLog2 of exponent fexp[block]
+    const int2 *b = reinterpret_cast<const int2 *>(buf) + fftc::pad(tid);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int2 x = b[272 * r];
+      const int e = fexp[(tid >> kFixBlkLog2) + r * (256 >> kFixBlkLog2)];
+      v[r] = fftc::mk((T)ldexpf((float)x.x, -e), (T)ldexpf((float)x.y, -e));
+    }
+  } else if (IN == 1) {
     // the input is the FP64 accumulator (cell c at double2 c), cleared as it
     // is read; the FP32 stages below then use its first 4352 * 8 bytes
     double2 *a = reinterpret_cast<double2 *>(buf) + tid;
@@ -678,12 +694,62 @@ template <typename T, int S> inline int planes_lds_bytes(const Geom &g, int byte
                                            : bytes;
 }
 
-// acc: the block is an FP64 accumulator (cell c at doubles 2c, 2c + 1)
-// instead of the padded FP32 buffer.
+// Accumulation of the spread in the LDS block (the default is the padded
+// buffer of the path's type with float atomics, which on this GPU take about
+// 100 cycles per wave instruction, where integer and FP64 LDS atomics are
+// about as cheap as stores):
+// kAccF64: an FP64 accumulator, cell c at doubles 2c, 2c + 1;
+// kAccFix: block floating point in the padded buffer, each component an int32
+// scaled by 2^e_b for the cells' block b of 2^kFixBlkLog2 (e_b from a bound on the
+// block's contributions, so every partial sum stays below 2^30: bound_row_part).
+enum { kAccNative = 0, kAccF64 = 1, kAccFix = 2 };
+
+// Bound pass of kAccFix for one record: the number of records reaching each
+// block (cnt[b]) and the largest |re| + |im| of one record there (mx[b], FP32
+// bits: ordered as unsigned).  Every component of a record's contribution to
+// a cell is at most (|re| + |im|) times a product of kernel taps, each at
+// most 1 (the ES kernel's maximum, e^0 at the centre, which its polynomial
+// fits to the requested accuracy), and twiddles have modulus 1; the factor 2
+// between the scaled bound (2^30) and the int32 range covers taps slightly
+// above 1 and the rounding.  A record's kw <= 16 cells fall in at most two
+// blocks (of 32 cells).
+template <typename T>
+__device__ __forceinline__ void bound_row_part(const Rec<T> rc, const Geom &g, int p, int row0, int mm,
+                                               unsigned *cnt, unsigned *mx) {
+  const int kw = g.kw, nf2 = g.nf2;
+  int k = p - rc.iw;
+  if (k < 0) k += g.P;
+  if (k >= g.kww) return;
+  int jj = row0 - rc.iy;
+  if (jj < 0) jj += nf2;
+  if (jj >= kw) return;
+  const unsigned bb = __float_as_uint(fabsf((float)rc.re) + fabsf((float)rc.im));
+  const int b0 = (rc.ix & mm) >> kFixBlkLog2, b1 = ((rc.ix + kw - 1) & mm) >> kFixBlkLog2;
+  atomicAdd(cnt + b0, 1u);
+  atomicMax(mx + b0, bb);
+  if (b1 != b0) {
+    atomicAdd(cnt + b1, 1u);
+    atomicMax(mx + b1, bb);
+  }
+}
+
+// kAccFix: block b's exponent e from (cnt, mx): with mx < 2^(L + 1) (L =
+// ilogb(mx)) and cnt <= 2^c, e = 29 - L - c keeps cnt mx 2^e < 2^30 (a margin
+// of 2 below the int32 range for taps slightly above 1 and the rounding).
+// Contributions are scaled by 2^e with ldexp, so data of any FP32 magnitude
+// keep 30 bits.  kFixNaN marks a block with a non-finite bound: the row
+// part's transform is then NaN throughout, as a non-finite input makes it.
+__device__ __forceinline__ int fix_block_exp(unsigned n, unsigned mb) {
+  if (n == 0 || mb == 0u) return 0;
+  if (mb >= 0x7f800000u) return kFixNaN;
+  const int c = n > 1 ? 32 - __clz((int)(n - 1)) : 0;
+  return 29 - ilogbf(__uint_as_float(mb)) - c;
+}
+
 template <typename T, int KWM, bool PAD = true, typename Tw>
 __device__ __forceinline__ void spread_row_part(const Rec<T> rc, const Taps<T, KWM> &tp, T *ldsf, const Tw &tv,
                                                 const cpx<T> *wh, const Geom &g, int p, int row0, int h, int mm,
-                                                int sign, bool acc = false) {
+                                                int sign, int accm = kAccNative, const unsigned *fixtab = nullptr) {
   const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2;
   int k = p - rc.iw;
   if (k < 0) k += g.P;
@@ -700,6 +766,17 @@ __device__ __forceinline__ void spread_row_part(const Rec<T> rc, const Taps<T, K
     vim = vre * ws + vim * w.x;
     vre = t;
   }
+  // kAccFix: the scales of the (at most two) blocks of the record's cells,
+  // from the block table (counts, then bounds)
+  int b0 = 0, e0 = 0, e1 = 0;
+  if (accm == kAccFix) {
+    b0 = (rc.ix & mm) >> kFixBlkLog2;
+    const int b1 = ((rc.ix + kw - 1) & mm) >> kFixBlkLog2;
+    e0 = fix_block_exp(fixtab[b0], fixtab[kFixBlks + b0]);
+    e1 = fix_block_exp(fixtab[b1], fixtab[kFixBlks + b1]);
+    e0 = e0 == kFixNaN ? 0 : e0;
+    e1 = e1 == kFixNaN ? 0 : e1;
+  }
 #pragma unroll
   for (int i = 0; i < KWM; ++i) {
     if (i < kw) {
@@ -711,10 +788,15 @@ __device__ __forceinline__ void spread_row_part(const Rec<T> rc, const Taps<T, K
         im = re * wh[i].y + im * wh[i].x;
         re = t;
       }
-      if (acc) {
+      if (accm == kAccF64) {
         double *d = reinterpret_cast<double *>(ldsf) + 2 * (cx & mm);
         atomicAdd(d, (double)re);
         atomicAdd(d + 1, (double)im);
+      } else if (accm == kAccFix) {
+        const int e = ((cx & mm) >> kFixBlkLog2) == b0 ? e0 : e1;
+        int *d = reinterpret_cast<int *>(ldsf) + 2 * (PAD ? fftc::pad(cx & mm) : (cx & mm));
+        atomicAdd(d, __float2int_rn(ldexpf((float)re, e)));
+        atomicAdd(d + 1, __float2int_rn(ldexpf((float)im, e)));
       } else {
         const int q = 2 * (PAD ? fftc::pad(cx & mm) : (cx & mm));
         atomicAdd(ldsf + q, re);
@@ -745,6 +827,11 @@ __global__ void __launch_bounds__(256) spread_row_part_planes_kernel(
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ WalkTable wt[2];
   __shared__ cpx<T> whs[KWM];
+  // kAccFix block table (FP32 split rows): counts and bounds, then the scales
+  constexpr bool kFixable = sizeof(T) == 4 && S > 1;
+  __shared__ unsigned fixt[kFixable ? 2 * kFixBlks : 1];
+  __shared__ int fixe[kFixable ? kFixBlks : 1];  // the blocks' exponents, for the first FFT stage
+  __shared__ int fixnan;                         // a block's bound is not finite
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
   T *ldsf = reinterpret_cast<T *>(smem_raw);
   const int nf1 = g.nf1, m = nf1 / S, nt = blockDim.x, sign = +1, j = threadIdx.x;
@@ -752,9 +839,15 @@ __global__ void __launch_bounds__(256) spread_row_part_planes_kernel(
   // consecutive items of one XCD; the XCDs take such blocks in turn
   const int item = xcd_interleave(blockIdx.x, gridDim.x, 4 * S);
   const int row0 = item / S, h = item % S;
-  // FP32 parts of 4096 accumulate in FP64 (LDS float atomics are slow), in a
-  // block of 4096 x 16 bytes that the FP32 transform then reuses
+  // FP32 parts of 4096 do not use LDS float atomics (slow): whole rows
+  // accumulate in FP64, in a block of 4096 x 16 bytes that the FP32 transform
+  // then reuses; split rows, at four workgroups per CU, in block floating
+  // point in the FP32 buffer itself (kAccFix)
   const bool regs = sizeof(T) == 4 && E == 16 && m == 4096, acc = planes_acc64<T, E, S>(m);
+  const bool fix = kFixable && regs && !acc;
+  const int accm = acc ? kAccF64 : (fix ? kAccFix : kAccNative);
+  if (kFixable && j < 2 * kFixBlks) fixt[j] = 0u;
+  if (kFixable && j == 0) fixnan = 0;
   const int ntot = acc ? kAcc64Bytes / (int)sizeof(cpx<T>) : fftc::padded_size(m);
   const auto tv = lds_twiddles<T>(lds + ntot, tw, nf1);
   if (j < 64) walk_table_build(wt[0], off, g, p0, row0, 1);
@@ -787,16 +880,29 @@ __global__ void __launch_bounds__(256) spread_row_part_planes_kernel(
     const bool more = pl + 1 < nb;
     WalkAhead wa;
     if (j < 64 && more) walk_issue(wa, off, g, p + 1, row0);
+    if (fix && total > 0) {
+      // block bounds (kAccFix); a plane without records leaves the table zero
+      if (j < total) bound_row_part<T>(r0, g, p, row0, m - 1, fixt, fixt + kFixBlks);
+      for (int t = j + nt; t < total; t += nt)
+        bound_row_part<T>(recs[walk_record(wt[cur], t)], g, p, row0, m - 1, fixt, fixt + kFixBlks);
+      __syncthreads();
+      if (j < kFixBlks) {
+        // published to the first FFT stage by the barrier after the spread
+        const int e = fix_block_exp(fixt[j], fixt[kFixBlks + j]);
+        fixe[j] = e == kFixNaN ? 0 : e;
+        if (e == kFixNaN) fixnan = 1;
+      }
+    }
     if (j < total) {
       Taps<T, KWM> q0;
       load_taps<T, KWM>(taps, idx0, q0);
-      spread_row_part<T, KWM>(r0, q0, ldsf, tv, whs, g, p, row0, h, m - 1, sign, acc);
+      spread_row_part<T, KWM>(r0, q0, ldsf, tv, whs, g, p, row0, h, m - 1, sign, accm, fixt);
     }
     for (int t = j + nt; t < total; t += nt) {
       const int ia = walk_record(wt[cur], t);
       Taps<T, KWM> ta;
       load_taps<T, KWM>(taps, ia, ta);
-      spread_row_part<T, KWM>(recs[ia], ta, ldsf, tv, whs, g, p, row0, h, m - 1, sign, acc);
+      spread_row_part<T, KWM>(recs[ia], ta, ldsf, tv, whs, g, p, row0, h, m - 1, sign, accm, fixt);
     }
     if (j < 64 && more) walk_finish(wt[cur ^ 1], wa);
     __syncthreads();  // spread done; the next plane's walk table is complete
@@ -808,7 +914,7 @@ __global__ void __launch_bounds__(256) spread_row_part_planes_kernel(
     cpx<T> *hb = H + h_plane(g, pl) + row0 * kTile;
     if (total == 0) {
       // no visibility reaches this part of this row and plane: its outputs are
-      // zero (and the LDS block still is)
+      // zero (and the LDS block and the block table still are)
       for (int i = j; i < nout; i += nt) hb[h_slot(g, hs + i)] = fftc::mk((T)0, (T)0);
       continue;
     }
@@ -818,10 +924,17 @@ __global__ void __launch_bounds__(256) spread_row_part_planes_kernel(
       // image column for r < 4 and r >= 12 only; its slot steps by 64 tiles
       // per r (as in row_ifft_interp_part_kernel's inputs)
       cpx<T> v[16];
+      const bool pnan = fix && fixnan;  // read before the transform's barriers; cleared after them
       if (acc)
-        fft4096_part_to_regs<T, true>(lds, v, sign, pt);
+        fft4096_part_to_regs<T, 1>(lds, v, sign, pt);
+      else if (fix)
+        fft4096_part_to_regs<T, 2>(lds, v, sign, pt, fixe);
       else
-        fft4096_part_to_regs<T, false>(lds, v, sign, pt);
+        fft4096_part_to_regs<T, 0>(lds, v, sign, pt);
+      if (pnan) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[r] = fftc::mk((T)NAN, (T)NAN);
+      }
       const int tid = threadIdx.x, q = (h + h1) / S, k0 = S * tid + h + h1;
       cpx<T> *a0 = hb + h_slot(g, hs + tid + q);
       const long estep = (long)(256 >> kTileLog2) * g.tstride, mstep = (long)(4096 >> kTileLog2) * g.tstride;
@@ -833,7 +946,9 @@ __global__ void __launch_bounds__(256) spread_row_part_planes_kernel(
           if (k0 + S * 256 * r >= nf1) a0[r * estep - mstep] = v[r];
         }
       }
-      __syncthreads();  // the block is clear before the next plane's spread
+      if (fix && j < 2 * kFixBlks) fixt[j] = 0u;  // last read by the spread
+      if (fix && j == 0) fixnan = 0;
+      __syncthreads();  // the block and the table are clear before the next plane's spread
     } else {
       lds_fft_part<T, E, 16, S>(lds, 31 - __clz(m), sign, tv);
       int tid = threadIdx.x;

@@ -508,6 +508,47 @@ def test_split_rows_dense_and_empty_rows():
     assert e3 < 2e-6 and e4 < 3 * eps + 3e-6
 
 
+def test_split_rows_block_scaling_over_magnitudes():
+    """The FP32 split-row spread accumulates each 32-cell block of a row part
+    as int32 scaled by a power of two chosen from a bound on the block's
+    records: the image must scale with the visibilities over the whole FP32
+    range (1e-36 .. 1e30, where a fixed scale would lose or overflow bits),
+    match the rocFFT path (FP32 float atomics) on data whose rows mix
+    magnitudes 1e-6 and 1e6, and turn a NaN visibility into NaNs."""
+    lib = _lib()
+    n, f0, eps = 8192, 1e9, 1e-6
+    say(f"[split rows, block scaling] fp32 {n}x{n}")
+    rng = np.random.default_rng(23)
+    dx = 1.0 * np.pi / 180 / n
+    nrow = 20000
+    uvw = np.empty((nrow, 3))
+    uvw[:, :2] = (rng.random((nrow, 2)) - 0.5) * 0.98 * (C / f0 / dx)
+    uvw[:, 2] = rng.uniform(-1, 1, nrow) * 30.0
+    wgt = np.ones(nrow)
+    vis = rng.standard_normal(nrow) + 1j * rng.standard_normal(nrow)
+    info = _lib_plan(lib, nrow, n, n, 1.0, f0, eps, uvw, "fp32")
+    assert info["fft_lds"] == 1 and info["rsplit"] == 4
+    d1 = np.asarray(gpu_f32(True, uvw, vis, wgt, n, n, 1.0, f0, eps))
+    for s in (1e-36, 1e30):
+        ds = np.asarray(gpu_f32(True, uvw, vis * s, wgt, n, n, 1.0, f0, eps))
+        e = O.l2error(d1.astype(np.complex128) * s, ds.astype(np.complex128))
+        say(f"  scale {s:g}: {e:.2e}")
+        assert np.all(np.isfinite(ds)) and e < 1e-6
+    mixed = vis * np.where(rng.random(nrow) < 0.5, 1e-6, 1e6)
+    assert lib.curig_set_fft_path(1) == 0
+    try:
+        d_roc = np.asarray(gpu_f32(True, uvw, mixed, wgt, n, n, 1.0, f0, eps))
+    finally:
+        lib.curig_set_fft_path(0)
+    d_lds = np.asarray(gpu_f32(True, uvw, mixed, wgt, n, n, 1.0, f0, eps))
+    e = O.l2error(d_roc, d_lds)
+    say(f"  mixed 1e-6 / 1e6 vs rocFFT path: {e:.2e}")
+    assert e < 2e-6
+    bad = vis.copy()
+    bad[nrow // 2] = np.nan
+    assert np.isnan(np.asarray(gpu_f32(True, uvw, bad, wgt, n, n, 1.0, f0, eps))).any()
+
+
 def _lib_plan(lib, nrow, nx, ny, fov, f0, eps, uvw, precision):
     from curig_amd import _lib as L
 

@@ -288,12 +288,12 @@ template <typename T> __device__ __forceinline__ void apply_tw16(cpx<T> *u, cons
 
 // Cells per block of the block-floating-point spread (kAccFix below) of a
 // 4096-cell row part, and the blocks.
-constexpr int kFixBlkLog2 = 5, kFixBlks = 4096 >> kFixBlkLog2;
+constexpr int kFixBlkLog2 = 4, kFixBlks = 4096 >> kFixBlkLog2;
 constexpr int kFixNaN = 0x40000000;  // exponent of a block with a non-finite bound
 
 template <typename T, int IN = 0>
 __device__ __forceinline__ void fft4096_part_to_regs(cpx<T> *buf, cpx<T> *v, int sign, const PartTw<T> &pt,
-                                                     const int *fexp = nullptr) {
+                                                     const short *fexp = nullptr) {
   // every stage's butterfly j reads x[j + 256 r] at pad(j) + 272 r
   int tid = threadIdx.x;
   __asm__ volatile("" : "+v"(tid));
@@ -307,17 +307,6 @@ __device__ __forceinline__ void fft4096_part_to_regs(cpx<T> *buf, cpx<T> *v, int
       const int e = fexp[(tid >> kFixBlkLog2) + r * (256 >> kFixBlkLog2)];
       v[r] = fftc::mk((T)ldexpf((float)x.x, -e), (T)ldexpf((float)x.y, -e));
     }
-  } else if (IN == 1) {
-    // the input is the FP64 accumulator (cell c at double2 c), cleared as it
-    // is read; the FP32 stages below then use its first 4352 * 8 bytes
-    double2 *a = reinterpret_cast<double2 *>(buf) + tid;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const double2 x = a[256 * r];
-      v[r] = fftc::mk((T)x.x, (T)x.y);
-    }
-#pragma unroll
-    for (int r = 0; r < 16; ++r) a[256 * r] = make_double2(0.0, 0.0);
   } else {
     const cpx<T> *b = buf + fftc::pad(tid);
 #pragma unroll
@@ -679,43 +668,27 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) row_ifft_interp_kernel(c
 // holds z_h[n mod m] = sum over the row's cells n of x[n] w^(sign h n), whose
 // m-point FFT gives the row transform at the indices S k + h (radix-S
 // decimation in frequency; w = e^{2 pi i / nf1}, m = nf1 / S).
-// The plane-loop row kernel's FP64 accumulation, for whole FP32 rows of 4096
-// (the register last stage): 4096 cells of 16 bytes, then the twiddle table.
-// Split rows keep FP32 accumulation: their workgroups walk fewer records per
-// item, and two workgroups per CU instead of four cost more than the atomics
-// save (16384^2 grids: 25.3 against 20.2 ms per launch; 4096^2: 1.72 against
-// 1.96 ms).
-constexpr int kAcc64Bytes = 4096 * 16;
-template <typename T, int E, int S> __host__ __device__ inline bool planes_acc64(int m) {
-  return sizeof(T) == 4 && E == 16 && S == 1 && m == 4096;
-}
-template <typename T, int S> inline int planes_lds_bytes(const Geom &g, int bytes) {
-  return planes_acc64<T, 16, S>(g.nf1 / S) ? kAcc64Bytes + fftc::twiddle_entries(g.nf1) * (int)sizeof(cpx<T>)
-                                           : bytes;
-}
+// Accumulation of the spread in the LDS block: kAccNative, the padded buffer
+// of the path's type with float atomics; kAccFix, block floating point in the
+// same buffer, each component an int32 scaled by 2^e_b for the cells' block b
+// of 2^kFixBlkLog2 (e_b from a bound on the block's contributions, so every
+// partial sum stays below 2^30: bound_row_part).  LDS FP32 atomics take about
+// 100 cycles per wave instruction on this GPU, integer ones about as little
+// as stores.
+enum { kAccNative = 0, kAccFix = 2 };
 
-// Accumulation of the spread in the LDS block (the default is the padded
-// buffer of the path's type with float atomics, which on this GPU take about
-// 100 cycles per wave instruction, where integer and FP64 LDS atomics are
-// about as cheap as stores):
-// kAccF64: an FP64 accumulator, cell c at doubles 2c, 2c + 1;
-// kAccFix: block floating point in the padded buffer, each component an int32
-// scaled by 2^e_b for the cells' block b of 2^kFixBlkLog2 (e_b from a bound on the
-// block's contributions, so every partial sum stays below 2^30: bound_row_part).
-enum { kAccNative = 0, kAccF64 = 1, kAccFix = 2 };
-
-// Bound pass of kAccFix for one record: the number of records reaching each
-// block (cnt[b]) and the largest |re| + |im| of one record there (mx[b], FP32
-// bits: ordered as unsigned).  Every component of a record's contribution to
-// a cell is at most (|re| + |im|) times a product of kernel taps, each at
+// Bound pass of kAccFix for one record (index idx): adds the record's bound
+// (|re| + |im|) |y tap| |z tap| for this row and plane to bsum[b] of each
+// block b its cells reach (at most two: kw <= 16).  Every component of the
+// record's contribution to a cell is at most that bound times an x tap, at
 // most 1 (the ES kernel's maximum, e^0 at the centre, which its polynomial
-// fits to the requested accuracy), and twiddles have modulus 1; the factor 2
-// between the scaled bound (2^30) and the int32 range covers taps slightly
-// above 1 and the rounding.  A record's kw <= 16 cells fall in at most two
-// blocks (of 32 cells).
-template <typename T>
-__device__ __forceinline__ void bound_row_part(const Rec<T> rc, const Geom &g, int p, int row0, int mm,
-                                               unsigned *cnt, unsigned *mx) {
+// fits to the requested accuracy), and twiddles have modulus 1; so bsum[b]
+// bounds every partial sum of the block's cells.  The factor 2 between the
+// scaled bound (2^30) and the int32 range covers x taps slightly above 1,
+// the FP32 rounding of the sum and that of the contributions.
+template <typename T, int KWM>
+__device__ __forceinline__ void bound_row_part(const Rec<T> rc, int idx, const T *__restrict__ taps, const Geom &g,
+                                               int p, int row0, int mm, float *bsum) {
   const int kw = g.kw, nf2 = g.nf2;
   int k = p - rc.iw;
   if (k < 0) k += g.P;
@@ -723,33 +696,29 @@ __device__ __forceinline__ void bound_row_part(const Rec<T> rc, const Geom &g, i
   int jj = row0 - rc.iy;
   if (jj < 0) jj += nf2;
   if (jj >= kw) return;
-  const unsigned bb = __float_as_uint(fabsf((float)rc.re) + fabsf((float)rc.im));
+  const T *tq = taps + (size_t)idx * (3 * KWM);
+  const float b = (fabsf((float)rc.re) + fabsf((float)rc.im)) * fabsf((float)tq[KWM + jj]) *
+                  fabsf((float)tq[2 * KWM + k]);
   const int b0 = (rc.ix & mm) >> kFixBlkLog2, b1 = ((rc.ix + kw - 1) & mm) >> kFixBlkLog2;
-  atomicAdd(cnt + b0, 1u);
-  atomicMax(mx + b0, bb);
-  if (b1 != b0) {
-    atomicAdd(cnt + b1, 1u);
-    atomicMax(mx + b1, bb);
-  }
+  atomicAdd(bsum + b0, b);
+  if (b1 != b0) atomicAdd(bsum + b1, b);
 }
 
-// kAccFix: block b's exponent e from (cnt, mx): with mx < 2^(L + 1) (L =
-// ilogb(mx)) and cnt <= 2^c, e = 29 - L - c keeps cnt mx 2^e < 2^30 (a margin
-// of 2 below the int32 range for taps slightly above 1 and the rounding).
-// Contributions are scaled by 2^e with ldexp, so data of any FP32 magnitude
-// keep 30 bits.  kFixNaN marks a block with a non-finite bound: the row
-// part's transform is then NaN throughout, as a non-finite input makes it.
-__device__ __forceinline__ int fix_block_exp(unsigned n, unsigned mb) {
-  if (n == 0 || mb == 0u) return 0;
-  if (mb >= 0x7f800000u) return kFixNaN;
-  const int c = n > 1 ? 32 - __clz((int)(n - 1)) : 0;
-  return 29 - ilogbf(__uint_as_float(mb)) - c;
+// kAccFix: the exponent e of a block from its bound B (< 2^(ilogb(B) + 1)):
+// e = 29 - ilogb(B) keeps B 2^e < 2^30.  Contributions are scaled by 2^e with
+// ldexp, so data of any FP32 magnitude keep 30 bits.  kFixNaN marks a block
+// with a non-finite bound: the row part's transform is then NaN throughout,
+// as a non-finite input makes it.
+__device__ __forceinline__ int fix_block_exp(float B) {
+  if (!(B < __uint_as_float(0x7f800000u))) return kFixNaN;  // NaN or +inf
+  if (B == 0.f) return 0;
+  return 29 - ilogbf(B);
 }
 
 template <typename T, int KWM, bool PAD = true, typename Tw>
 __device__ __forceinline__ void spread_row_part(const Rec<T> rc, const Taps<T, KWM> &tp, T *ldsf, const Tw &tv,
                                                 const cpx<T> *wh, const Geom &g, int p, int row0, int h, int mm,
-                                                int sign, int accm = kAccNative, const unsigned *fixtab = nullptr) {
+                                                int sign, int accm = kAccNative, const float *fixtab = nullptr) {
   const int kw = g.kw, nf1 = g.nf1, nf2 = g.nf2;
   int k = p - rc.iw;
   if (k < 0) k += g.P;
@@ -766,14 +735,14 @@ __device__ __forceinline__ void spread_row_part(const Rec<T> rc, const Taps<T, K
     vim = vre * ws + vim * w.x;
     vre = t;
   }
-  // kAccFix: the scales of the (at most two) blocks of the record's cells,
-  // from the block table (counts, then bounds)
+  // kAccFix: the exponents of the (at most two) blocks of the record's cells,
+  // from the blocks' bounds
   int b0 = 0, e0 = 0, e1 = 0;
   if (accm == kAccFix) {
     b0 = (rc.ix & mm) >> kFixBlkLog2;
     const int b1 = ((rc.ix + kw - 1) & mm) >> kFixBlkLog2;
-    e0 = fix_block_exp(fixtab[b0], fixtab[kFixBlks + b0]);
-    e1 = fix_block_exp(fixtab[b1], fixtab[kFixBlks + b1]);
+    e0 = fix_block_exp(fixtab[b0]);
+    e1 = fix_block_exp(fixtab[b1]);
     e0 = e0 == kFixNaN ? 0 : e0;
     e1 = e1 == kFixNaN ? 0 : e1;
   }
@@ -788,11 +757,7 @@ __device__ __forceinline__ void spread_row_part(const Rec<T> rc, const Taps<T, K
         im = re * wh[i].y + im * wh[i].x;
         re = t;
       }
-      if (accm == kAccF64) {
-        double *d = reinterpret_cast<double *>(ldsf) + 2 * (cx & mm);
-        atomicAdd(d, (double)re);
-        atomicAdd(d + 1, (double)im);
-      } else if (accm == kAccFix) {
+      if (accm == kAccFix) {
         const int e = ((cx & mm) >> kFixBlkLog2) == b0 ? e0 : e1;
         int *d = reinterpret_cast<int *>(ldsf) + 2 * (PAD ? fftc::pad(cx & mm) : (cx & mm));
         atomicAdd(d, __float2int_rn(ldexpf((float)re, e)));
@@ -818,8 +783,9 @@ __device__ __forceinline__ void spread_row_part(const Rec<T> rc, const Taps<T, K
 // first record of it is in flight, and the twiddle table and the part's
 // factors are loaded once (c3: 23.8 against 24.3 ms per 32-plane launch for
 // one workgroup per (row, part, plane)).  With S = 1 it runs whole FP32 rows
-// of 4096 (2048^2 images: 1.96 against 2.07 ms per launch for one workgroup
-// per (row, plane)).
+// of 4096 (2048^2 images).  FP32 parts of 4096 accumulate the spread in block
+// floating point (kAccFix): int32 LDS atomics instead of float ones, after a
+// bound pass over the plane's records.
 template <typename T, int E, int KWM, int S>
 __global__ void __launch_bounds__(256) spread_row_part_planes_kernel(
     const Rec<T> *__restrict__ recs, const T *__restrict__ taps, const int *__restrict__ off, cpx<T> *__restrict__ H,
@@ -827,10 +793,11 @@ __global__ void __launch_bounds__(256) spread_row_part_planes_kernel(
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ WalkTable wt[2];
   __shared__ cpx<T> whs[KWM];
-  // kAccFix block table (FP32 split rows): counts and bounds, then the scales
-  constexpr bool kFixable = sizeof(T) == 4 && S > 1;
-  __shared__ unsigned fixt[kFixable ? 2 * kFixBlks : 1];
-  __shared__ int fixe[kFixable ? kFixBlks : 1];  // the blocks' exponents, for the first FFT stage
+  // kAccFix block table (FP32): the blocks' bounds, and their exponents for
+  // the first FFT stage
+  constexpr bool kFixable = sizeof(T) == 4;
+  __shared__ float fixt[kFixable ? kFixBlks : 1];
+  __shared__ short fixe[kFixable ? kFixBlks : 1];
   __shared__ int fixnan;                         // a block's bound is not finite
   cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
   T *ldsf = reinterpret_cast<T *>(smem_raw);
@@ -839,16 +806,13 @@ __global__ void __launch_bounds__(256) spread_row_part_planes_kernel(
   // consecutive items of one XCD; the XCDs take such blocks in turn
   const int item = xcd_interleave(blockIdx.x, gridDim.x, 4 * S);
   const int row0 = item / S, h = item % S;
-  // FP32 parts of 4096 do not use LDS float atomics (slow): whole rows
-  // accumulate in FP64, in a block of 4096 x 16 bytes that the FP32 transform
-  // then reuses; split rows, at four workgroups per CU, in block floating
-  // point in the FP32 buffer itself (kAccFix)
-  const bool regs = sizeof(T) == 4 && E == 16 && m == 4096, acc = planes_acc64<T, E, S>(m);
-  const bool fix = kFixable && regs && !acc;
-  const int accm = acc ? kAccF64 : (fix ? kAccFix : kAccNative);
-  if (kFixable && j < 2 * kFixBlks) fixt[j] = 0u;
+  // FP32 parts of 4096 accumulate in block floating point (kAccFix), not
+  // with LDS float atomics
+  const bool regs = sizeof(T) == 4 && E == 16 && m == 4096, fix = kFixable && regs;
+  const int accm = fix ? kAccFix : kAccNative;
+  if (kFixable && j < kFixBlks) fixt[j] = 0.f;
   if (kFixable && j == 0) fixnan = 0;
-  const int ntot = acc ? kAcc64Bytes / (int)sizeof(cpx<T>) : fftc::padded_size(m);
+  const int ntot = fftc::padded_size(m);
   const auto tv = lds_twiddles<T>(lds + ntot, tw, nf1);
   if (j < 64) walk_table_build(wt[0], off, g, p0, row0, 1);
   for (int i = j; i < ntot; i += nt) lds[i] = fftc::mk((T)0, (T)0);
@@ -882,14 +846,16 @@ __global__ void __launch_bounds__(256) spread_row_part_planes_kernel(
     if (j < 64 && more) walk_issue(wa, off, g, p + 1, row0);
     if (fix && total > 0) {
       // block bounds (kAccFix); a plane without records leaves the table zero
-      if (j < total) bound_row_part<T>(r0, g, p, row0, m - 1, fixt, fixt + kFixBlks);
-      for (int t = j + nt; t < total; t += nt)
-        bound_row_part<T>(recs[walk_record(wt[cur], t)], g, p, row0, m - 1, fixt, fixt + kFixBlks);
+      if (j < total) bound_row_part<T, KWM>(r0, idx0, taps, g, p, row0, m - 1, fixt);
+      for (int t = j + nt; t < total; t += nt) {
+        const int ia = walk_record(wt[cur], t);
+        bound_row_part<T, KWM>(recs[ia], ia, taps, g, p, row0, m - 1, fixt);
+      }
       __syncthreads();
       if (j < kFixBlks) {
         // published to the first FFT stage by the barrier after the spread
-        const int e = fix_block_exp(fixt[j], fixt[kFixBlks + j]);
-        fixe[j] = e == kFixNaN ? 0 : e;
+        const int e = fix_block_exp(fixt[j]);
+        fixe[j] = (short)(e == kFixNaN ? 0 : e);
         if (e == kFixNaN) fixnan = 1;
       }
     }
@@ -925,9 +891,7 @@ __global__ void __launch_bounds__(256) spread_row_part_planes_kernel(
       // per r (as in row_ifft_interp_part_kernel's inputs)
       cpx<T> v[16];
       const bool pnan = fix && fixnan;  // read before the transform's barriers; cleared after them
-      if (acc)
-        fft4096_part_to_regs<T, 1>(lds, v, sign, pt);
-      else if (fix)
+      if (fix)
         fft4096_part_to_regs<T, 2>(lds, v, sign, pt, fixe);
       else
         fft4096_part_to_regs<T, 0>(lds, v, sign, pt);
@@ -946,7 +910,7 @@ __global__ void __launch_bounds__(256) spread_row_part_planes_kernel(
           if (k0 + S * 256 * r >= nf1) a0[r * estep - mstep] = v[r];
         }
       }
-      if (fix && j < 2 * kFixBlks) fixt[j] = 0u;  // last read by the spread
+      if (fix && j < kFixBlks) fixt[j] = 0.f;  // last read by the spread
       if (fix && j == 0) fixnan = 0;
       __syncthreads();  // the block and the table are clear before the next plane's spread
     } else {
@@ -1570,7 +1534,7 @@ void lds_spread_rows(const Rec<T> *recs, const T *taps, const int *off, const Ge
                      const cpx<T> *tw1, hipStream_t st) {
   if (g.rsplit == kRowSplit) {
     const int nt = g.nf1 / (16 * kRowSplit), grd = kRowSplit * g.nf2;
-    const int lb = planes_lds_bytes<T, kRowSplit>(g, g.lds_rowp_bytes);
+    const int lb = g.lds_rowp_bytes;
     if (g.kw <= 8)
       spread_row_part_planes_kernel<T, 16, 8, kRowSplit><<<grd, nt, lb, st>>>(recs, taps, off, H, tw1, g, p0, nb);
     else
@@ -1579,7 +1543,7 @@ void lds_spread_rows(const Rec<T> *recs, const T *taps, const int *off, const Ge
   }
   if (whole_row_planes<T>(g)) {
     // FP32 rows of 4096: one workgroup per row walks the batch's planes
-    const int lb = planes_lds_bytes<T, 1>(g, g.lds_row_bytes);
+    const int lb = g.lds_row_bytes;
     if (g.kw <= 8)
       spread_row_part_planes_kernel<T, 16, 8, 1><<<g.nf2, 256, lb, st>>>(recs, taps, off, H, tw1, g, p0, nb);
     else
@@ -1599,12 +1563,12 @@ int allow_spread_rows(const Geom &g) {
   CURIG_TRY(allow_lds(spread_row_fft_kernel<T, 16, 8>, g.lds_row_bytes));
   CURIG_TRY(allow_lds(spread_row_fft_kernel<T, 16, 16>, g.lds_row_bytes));
   if (whole_row_planes<T>(g)) {
-    const int lb = planes_lds_bytes<T, 1>(g, g.lds_row_bytes);
+    const int lb = g.lds_row_bytes;
     CURIG_TRY((allow_lds(spread_row_part_planes_kernel<T, 16, 8, 1>, lb)));
     CURIG_TRY((allow_lds(spread_row_part_planes_kernel<T, 16, 16, 1>, lb)));
   }
   if (g.rsplit == kRowSplit) {
-    const int lb = planes_lds_bytes<T, kRowSplit>(g, g.lds_rowp_bytes);
+    const int lb = g.lds_rowp_bytes;
     CURIG_TRY((allow_lds(spread_row_part_planes_kernel<T, 16, 8, kRowSplit>, lb)));
     CURIG_TRY((allow_lds(spread_row_part_planes_kernel<T, 16, 16, kRowSplit>, lb)));
   }

@@ -508,16 +508,18 @@ def test_split_rows_dense_and_empty_rows():
     assert e3 < 2e-6 and e4 < 3 * eps + 3e-6
 
 
-def test_split_rows_block_scaling_over_magnitudes():
-    """The FP32 split-row spread accumulates each 32-cell block of a row part
-    as int32 scaled by a power of two chosen from a bound on the block's
-    records: the image must scale with the visibilities over the whole FP32
-    range (1e-36 .. 1e30, where a fixed scale would lose or overflow bits),
-    match the rocFFT path (FP32 float atomics) on data whose rows mix
-    magnitudes 1e-6 and 1e6, and turn a NaN visibility into NaNs."""
+@pytest.mark.parametrize("n", [2048, 8192])
+def test_row_spread_block_scaling_over_magnitudes(n):
+    """The FP32 row spread (4096-point rows whole at 2048^2, row parts of 4096
+    at 8192^2) accumulates each 32-cell block as int32 scaled by a power of
+    two chosen from a bound on the block's records: the image must scale with
+    the visibilities over the whole FP32 range (1e-36 .. 1e30, where a fixed
+    scale would lose or overflow bits), match the rocFFT path (FP32 float
+    atomics) on data whose rows mix magnitudes 1e-6 and 1e6, and turn a NaN
+    visibility into NaNs."""
     lib = _lib()
-    n, f0, eps = 8192, 1e9, 1e-6
-    say(f"[split rows, block scaling] fp32 {n}x{n}")
+    f0, eps = 1e9, 1e-6
+    say(f"[row spread, block scaling] fp32 {n}x{n}")
     rng = np.random.default_rng(23)
     dx = 1.0 * np.pi / 180 / n
     nrow = 20000
@@ -527,7 +529,7 @@ def test_split_rows_block_scaling_over_magnitudes():
     wgt = np.ones(nrow)
     vis = rng.standard_normal(nrow) + 1j * rng.standard_normal(nrow)
     info = _lib_plan(lib, nrow, n, n, 1.0, f0, eps, uvw, "fp32")
-    assert info["fft_lds"] == 1 and info["rsplit"] == 4
+    assert info["fft_lds"] == 1 and info["rsplit"] == (4 if n == 8192 else 0)
     d1 = np.asarray(gpu_f32(True, uvw, vis, wgt, n, n, 1.0, f0, eps))
     for s in (1e-36, 1e30):
         ds = np.asarray(gpu_f32(True, uvw, vis * s, wgt, n, n, 1.0, f0, eps))

@@ -286,6 +286,52 @@ template <typename T> __device__ __forceinline__ void apply_tw16(cpx<T> *u, cons
   }
 }
 
+// A 4096-point part (256 threads) from registers to LDS: v[r] holds
+// x[j + 256 r] for thread j (the first stage's butterfly), the stage
+// twiddles come from pt (part_twiddles), and X[k] ends at pad(k) in buf.
+// The caller has made buf free (a barrier after its last reader); the
+// function ends with a barrier.
+template <typename T>
+__device__ __forceinline__ void fft4096_part_regs_to_lds(cpx<T> *buf, cpx<T> *v, int sign, const PartTw<T> &pt) {
+  int tid = threadIdx.x;
+  __asm__ volatile("" : "+v"(tid));
+  fftc::dft_small<T, 16>(v, sign);
+  {
+    cpx<T> *b = buf + 17 * tid;  // stage 1 outputs: pad(16 j + r) = 17 j + r
+#pragma unroll
+    for (int r = 0; r < 16; ++r) b[r] = v[r];
+  }
+  __syncthreads();
+  int tid2 = threadIdx.x;
+  __asm__ volatile("" : "+v"(tid2));
+  {
+    const cpx<T> *b = buf + fftc::pad(tid2);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = b[272 * r];
+  }
+  apply_tw16(v, pt.tw2);
+  fftc::dft_small<T, 16>(v, sign);
+  __syncthreads();
+  {
+    const int k = tid2 & 15;
+    cpx<T> *b = buf + fftc::pad(16 * (tid2 - k) + k);  // stage 2 outputs: pad(d) + 17 r
+#pragma unroll
+    for (int r = 0; r < 16; ++r) b[17 * r] = v[r];
+  }
+  __syncthreads();
+  int tid3 = threadIdx.x;
+  __asm__ volatile("" : "+v"(tid3));
+  cpx<T> *own = buf + fftc::pad(tid3);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) v[r] = own[272 * r];
+  apply_tw16(v, pt.tw3);
+  fftc::dft_small<T, 16>(v, sign);
+  // X[j + 256 r] goes where this thread read its stage-3 inputs
+#pragma unroll
+  for (int r = 0; r < 16; ++r) own[272 * r] = v[r];
+  __syncthreads();
+}
+
 // Cells per block of the block-floating-point spread (kAccFix below) of a
 // 4096-cell row part, and the blocks.
 constexpr int kFixBlkLog2 = 4, kFixBlks = 4096 >> kFixBlkLog2;
@@ -632,13 +678,8 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) row_ifft_interp_kernel(c
   }
   const auto tv = lds_twiddles<T>(lds + fftc::padded_size(rpw * nf1), tw, nf1);
   if (threadIdx.x < 64) walk_table_build(wt, off, g, p, row0, rpw);
-  // FP32 rows of 4096 (256 threads): v is the first radix-16 stage's
-  // butterfly, so it runs in registers before the first LDS store
-  const bool regs = sizeof(T) == 4 && KWM <= 8 && E == 16 && rpw == 1 && nf1 == 4096;
-  if (!regs) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) lds[fftc::pad(threadIdx.x + e * nt)] = v[e];
-  }
+  for (int e = 0; e < E; ++e) lds[fftc::pad(threadIdx.x + e * nt)] = v[e];
   __syncthreads();
   const int total = wt.total;
   if (total == 0) return;  // no visibility reaches these rows of this plane
@@ -651,10 +692,7 @@ __global__ void __launch_bounds__(MaxThreads<T>::value) row_ifft_interp_kernel(c
     r0 = recs[idx0];
     load_taps<T, KWM>(taps, idx0, q0);
   }
-  if (regs)
-    part_stages_from_regs<T, E, 16, 4096, 1>(lds, v, sign, tv);
-  else
-    lds_fft_rows<T, E, 16>(lds, lg, rpw, sign, tv);
+  lds_fft_rows<T, E, 16>(lds, lg, rpw, sign, tv);
   if (threadIdx.x < total) interp_rows<T, KWM>(r0, q0, idx0, lds, g, p, row0, vacc);
   for (int t = threadIdx.x + nt; t < total; t += nt) {
     const int idx = walk_record(wt, t);
@@ -1046,6 +1084,88 @@ __global__ void __launch_bounds__(256) row_ifft_interp_part_kernel(
     Taps<T, KWM> q;
     load_taps<T, KWM>(taps, idx, q);
     interp_row_part<T, KWM>(recs[idx], q, idx, lds, tv, wh, g, p, row0, h, m - 1, sign, vacc);
+  }
+}
+
+// dirty2ms row pass for FP32 rows or row parts of 4096 (256 threads): as
+// row_ifft_interp_part_kernel (S = 1: whole rows), but the workgroup keeps
+// its (row, part) and walks the batch's nb planes, as the ms2dirty row kernel
+// does: the twiddle table, the part's tap factors and the thread's stage
+// twiddles are set up once, and while plane p is transformed the walk table
+// of plane p + 1 is in flight and each thread's first record of plane p is
+// loaded.
+template <typename T, int KWM, int S>
+__global__ void __launch_bounds__(256) row_ifft_interp_part_planes_kernel(
+    const cpx<T> *__restrict__ H, const Rec<T> *__restrict__ recs, const T *__restrict__ taps,
+    const int *__restrict__ off, double2 *__restrict__ vacc, const cpx<T> *__restrict__ tw, Geom g, int p0, int nb,
+    int sign) {
+  constexpr int E = 16, m = 4096;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ WalkTable wt[2];
+  __shared__ cpx<T> whs[KWM];
+  cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
+  const int nf1 = g.nf1, N1 = g.N1, nt = blockDim.x, j = threadIdx.x;
+  const int item = xcd_interleave(blockIdx.x, gridDim.x, 4 * S);
+  const int row0 = item / S, h = item % S;
+  const auto tv = lds_twiddles<T>(lds + fftc::padded_size(m), tw, nf1);
+  if (j < 64) walk_table_build(wt[0], off, g, p0, row0, 1);
+  if (j < KWM) {
+    // w^(sign h i) for the taps i, from the global table (the LDS copy is being written)
+    const int lb = fftc::twiddle_split(nf1);
+    const fftc::TwoLevel<T, const cpx<T> *> tg{tw, tw + (nf1 >> lb), lb};
+    const cpx<T> w = tg[h * j];
+    whs[j] = fftc::mk(w.x, (T)sign * w.y);
+  }
+  __syncthreads();
+  PartTw<T> pt;
+  part_twiddles(pt, tv, nf1, sign);
+  // input e of this thread is X[S i + h], i = j + nt e; only the first and
+  // last quarters can be image columns (as in row_ifft_interp_part_kernel)
+  const int h1 = N1 / 2, q = (h + h1) / S, k0 = S * j + h + h1;
+  const long estep = (long)(nt >> kTileLog2) * g.tstride, mstep = (long)(m >> kTileLog2) * g.tstride;
+  const long a0off = (long)row0 * kTile + h_slot(g, (h << g.slot_lg) + j + q);
+  int cur = 0;
+  for (int pl = 0; pl < nb; ++pl, cur ^= 1) {
+    const int p = p0 + pl;
+    const bool more = pl + 1 < nb;
+    cpx<T> v[E];
+    {
+      const cpx<T> *a0 = H + h_plane(g, pl) + a0off;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        v[e] = fftc::mk((T)0, (T)0);
+        if (e < E / 4) {
+          if (k0 + S * nt * e < N1) v[e] = a0[e * estep];
+        } else if (e >= 3 * E / 4) {
+          if (k0 + S * nt * e >= nf1) v[e] = a0[e * estep - mstep];
+        }
+      }
+    }
+    WalkAhead wa;
+    if (j < 64 && more) walk_issue(wa, off, g, p + 1, row0);
+    __syncthreads();  // the previous plane's interpolation is done with the LDS; this plane's walk table is complete
+    const int total = wt[cur].total;
+    if (total > 0) {
+      int idx0 = 0;
+      Rec<T> r0;
+      if (j < total) {
+        idx0 = walk_record(wt[cur], j);
+        r0 = recs[idx0];
+      }
+      fft4096_part_regs_to_lds(lds, v, sign, pt);
+      if (j < total) {
+        Taps<T, KWM> q0;
+        load_taps<T, KWM>(taps, idx0, q0);
+        interp_row_part<T, KWM>(r0, q0, idx0, lds, tv, whs, g, p, row0, h, m - 1, sign, vacc);
+      }
+      for (int t = j + nt; t < total; t += nt) {
+        const int idx = walk_record(wt[cur], t);
+        Taps<T, KWM> qq;
+        load_taps<T, KWM>(taps, idx, qq);
+        interp_row_part<T, KWM>(recs[idx], qq, idx, lds, tv, whs, g, p, row0, h, m - 1, sign, vacc);
+      }
+    }
+    if (j < 64 && more) walk_finish(wt[cur ^ 1], wa);
   }
 }
 
@@ -1575,10 +1695,35 @@ int allow_spread_rows(const Geom &g) {
   return kOk;
 }
 
+// FP32 rows (rsplit 0) or row parts (rsplit kRowSplit) of 4096 points
+template <typename T> inline bool interp_planes(const Geom &g) {
+  return sizeof(T) == 4 && ((g.rsplit == 0 && g.rpw == 1 && g.nf1 == 4096) ||
+                            (g.rsplit == kRowSplit && g.nf1 / kRowSplit == 4096));
+}
+
 // dirty2ms, one batch on the LDS path: inverse row FFT fused with the interpolation
 template <typename T>
 void lds_row_interp(const Rec<T> *recs, const T *taps, const int *off, const cpx<T> *H, double2 *vacc,
                     const cpx<T> *tw1, const Geom &g, int p0, int nb, hipStream_t st) {
+  if (interp_planes<T>(g)) {
+    // FP32 rows or row parts of 4096: one workgroup per (row, part) walks the batch's planes
+    const int S = g.rsplit == kRowSplit ? kRowSplit : 1;
+    const int lb = S == 1 ? g.lds_row_bytes : g.lds_rowp_bytes, grd = S * g.nf2;
+    if (S == 1) {
+      if (g.kw <= 8)
+        row_ifft_interp_part_planes_kernel<T, 8, 1><<<grd, 256, lb, st>>>(H, recs, taps, off, vacc, tw1, g, p0, nb, -1);
+      else
+        row_ifft_interp_part_planes_kernel<T, 16, 1><<<grd, 256, lb, st>>>(H, recs, taps, off, vacc, tw1, g, p0, nb, -1);
+    } else {
+      if (g.kw <= 8)
+        row_ifft_interp_part_planes_kernel<T, 8, kRowSplit><<<grd, 256, lb, st>>>(H, recs, taps, off, vacc, tw1, g, p0, nb,
+                                                                                -1);
+      else
+        row_ifft_interp_part_planes_kernel<T, 16, kRowSplit><<<grd, 256, lb, st>>>(H, recs, taps, off, vacc, tw1, g, p0,
+                                                                                 nb, -1);
+    }
+    return;
+  }
   if (g.rsplit == kRowSplit) {
     const dim3 grd(kRowSplit * g.nf2, nb);
     const int nt = g.nf1 / (16 * kRowSplit), lb = g.lds_rowp_bytes;
@@ -1603,6 +1748,14 @@ int allow_row_interp(const Geom &g) {
   if (g.rsplit == kRowSplit) {
     CURIG_TRY((allow_lds(row_ifft_interp_part_kernel<T, 16, 8, kRowSplit>, g.lds_rowp_bytes)));
     CURIG_TRY((allow_lds(row_ifft_interp_part_kernel<T, 16, 16, kRowSplit>, g.lds_rowp_bytes)));
+  }
+  if (interp_planes<T>(g)) {
+    const bool s1 = g.rsplit != kRowSplit;
+    const int lb = s1 ? g.lds_row_bytes : g.lds_rowp_bytes;
+    CURIG_TRY((allow_lds(row_ifft_interp_part_planes_kernel<T, 8, 1>, lb)));
+    CURIG_TRY((allow_lds(row_ifft_interp_part_planes_kernel<T, 16, 1>, lb)));
+    CURIG_TRY((allow_lds(row_ifft_interp_part_planes_kernel<T, 8, kRowSplit>, lb)));
+    CURIG_TRY((allow_lds(row_ifft_interp_part_planes_kernel<T, 16, kRowSplit>, lb)));
   }
   return kOk;
 }

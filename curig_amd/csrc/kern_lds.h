@@ -819,7 +819,7 @@ __device__ __forceinline__ void spread_row_part(const Rec<T> rc, const Taps<T, K
 // and walks the batch's nb planes, as the column kernels do: while plane p is
 // transformed, the walk table of plane p + 1 is complete and each thread's
 // first record of it is in flight, and the twiddle table and the part's
-// factors are loaded once (c3: 23.8 against 24.3 ms per 32-plane launch for
+// factors are loaded once (16384^2 grids: 23.8 against 24.3 ms per 32-plane launch for
 // one workgroup per (row, part, plane)).  With S = 1 it runs whole FP32 rows
 // of 4096 (2048^2 images).  FP32 parts of 4096 accumulate the spread in block
 // floating point (kAccFix): int32 LDS atomics instead of float ones, after a

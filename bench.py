@@ -794,10 +794,10 @@ def main():
         reg = "_reg" if stats_by_ch[0]["nf2"] == 16384 else ""
         kernels[f"col_fwd{reg}_kernel"] = (kb["col_fwd"], mean("t_fft", "m2d"))
         kernels[f"col_inv{reg}_kernel"] = (kb["col_inv"], mean("t_fft", "d2m"))
-        # FP32 rows and row parts of 4096 run the plane-loop interpolation kernel
+        # rows and row parts of 4096 run the plane-loop interpolation kernel
         nf1 = stats_by_ch[0]["nf1"]
         rs = stats_by_ch[0]["rsplit"]
-        ipl = run.prec == 1 and nf1 // max(rs, 1) == 4096
+        ipl = nf1 // max(rs, 1) == 4096
         kernels["row_ifft_interp_part_planes_kernel" if ipl else f"row_ifft_interp{part}_kernel"] = (
             kb["row_interp"], mean("t_grid", "d2m"))
     per_kernel = {k: {"bytes_per_launch": round(b / launches), "avg_launch_ms": round(t / launches * 1e3, 3),

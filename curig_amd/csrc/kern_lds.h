@@ -1087,7 +1087,7 @@ __global__ void __launch_bounds__(256) row_ifft_interp_part_kernel(
   }
 }
 
-// dirty2ms row pass for FP32 rows or row parts of 4096 (256 threads): as
+// dirty2ms row pass for rows or row parts of 4096 (256 threads): as
 // row_ifft_interp_part_kernel (S = 1: whole rows), but the workgroup keeps
 // its (row, part) and walks the batch's nb planes, as the ms2dirty row kernel
 // does: the twiddle table, the part's tap factors and the thread's stage
@@ -1695,10 +1695,10 @@ int allow_spread_rows(const Geom &g) {
   return kOk;
 }
 
-// FP32 rows (rsplit 0) or row parts (rsplit kRowSplit) of 4096 points
+// rows (rsplit 0) or row parts (rsplit kRowSplit) of 4096 points (FP64:
+// 173 VGPRs at the two workgroups per CU that its LDS allows)
 template <typename T> inline bool interp_planes(const Geom &g) {
-  return sizeof(T) == 4 && ((g.rsplit == 0 && g.rpw == 1 && g.nf1 == 4096) ||
-                            (g.rsplit == kRowSplit && g.nf1 / kRowSplit == 4096));
+  return (g.rsplit == 0 && g.rpw == 1 && g.nf1 == 4096) || (g.rsplit == kRowSplit && g.nf1 / kRowSplit == 4096);
 }
 
 // dirty2ms, one batch on the LDS path: inverse row FFT fused with the interpolation
@@ -1706,7 +1706,7 @@ template <typename T>
 void lds_row_interp(const Rec<T> *recs, const T *taps, const int *off, const cpx<T> *H, double2 *vacc,
                     const cpx<T> *tw1, const Geom &g, int p0, int nb, hipStream_t st) {
   if (interp_planes<T>(g)) {
-    // FP32 rows or row parts of 4096: one workgroup per (row, part) walks the batch's planes
+    // rows or row parts of 4096: one workgroup per (row, part) walks the batch's planes
     const int S = g.rsplit == kRowSplit ? kRowSplit : 1;
     const int lb = S == 1 ? g.lds_row_bytes : g.lds_rowp_bytes, grd = S * g.nf2;
     if (S == 1) {

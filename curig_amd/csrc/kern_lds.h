@@ -1645,7 +1645,7 @@ int allow_lds(K kernel, int bytes) {
 inline int col_groups(const Geom &g, int cpw) { return (g.nslot + cpw - 1) / cpw; }
 
 template <typename T> inline bool whole_row_planes(const Geom &g) {
-  return sizeof(T) == 4 && g.rsplit == 0 && g.rpw == 1 && g.nf1 == 4096;
+  return g.rsplit == 0 && g.rpw == 1 && g.nf1 == 4096;
 }
 
 // ms2dirty, one batch on the LDS path: spread + row FFT -> H (column-tiled)
@@ -1662,7 +1662,7 @@ void lds_spread_rows(const Rec<T> *recs, const T *taps, const int *off, const Ge
     return;
   }
   if (whole_row_planes<T>(g)) {
-    // FP32 rows of 4096: one workgroup per row walks the batch's planes
+    // rows of 4096: one workgroup per row walks the batch's planes
     const int lb = g.lds_row_bytes;
     if (g.kw <= 8)
       spread_row_part_planes_kernel<T, 16, 8, 1><<<g.nf2, 256, lb, st>>>(recs, taps, off, H, tw1, g, p0, nb);

@@ -846,7 +846,7 @@ __global__ void __launch_bounds__(256) spread_row_part_planes_kernel(
   const int row0 = item / S, h = item % S;
   // FP32 parts of 4096 accumulate in block floating point (kAccFix), not
   // with LDS float atomics
-  const bool regs = sizeof(T) == 4 && E == 16 && m == 4096, fix = kFixable && regs;
+  const bool regs = E == 16 && m == 4096, fix = kFixable && regs;
   const int accm = fix ? kAccFix : kAccNative;
   if (kFixable && j < kFixBlks) fixt[j] = 0.f;
   if (kFixable && j == 0) fixnan = 0;
@@ -923,7 +923,7 @@ __global__ void __launch_bounds__(256) spread_row_part_planes_kernel(
       continue;
     }
     if (regs) {
-      // last stage in registers (FP32: FP64 would hold 32 more registers): output X[j + 256 r] (part index) is image
+      // last stage in registers: output X[j + 256 r] (part index) is image
       // column S (j + 256 r) + h + N1 / 2 (folded by nf1), which can be an
       // image column for r < 4 and r >= 12 only; its slot steps by 64 tiles
       // per r (as in row_ifft_interp_part_kernel's inputs)

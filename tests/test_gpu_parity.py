@@ -511,7 +511,7 @@ def test_split_rows_dense_and_empty_rows():
 @pytest.mark.parametrize("n", [2048, 8192])
 def test_row_spread_block_scaling_over_magnitudes(n):
     """The FP32 row spread (4096-point rows whole at 2048^2, row parts of 4096
-    at 8192^2) accumulates each 32-cell block as int32 scaled by a power of
+    at 8192^2) accumulates each 16-cell block as int32 scaled by a power of
     two chosen from a bound on the block's records: the image must scale with
     the visibilities over the whole FP32 range (1e-36 .. 1e30, where a fixed
     scale would lose or overflow bits), match the rocFFT path (FP32 float

@@ -551,6 +551,43 @@ def test_row_spread_block_scaling_over_magnitudes(n):
     assert np.isnan(np.asarray(gpu_f32(True, uvw, bad, wgt, n, n, 1.0, f0, eps))).any()
 
 
+@pytest.mark.parametrize("precision,n", [("fp64", 2048), ("fp32", 2048), ("fp64", 8192), ("fp32", 8192)])
+def test_wide_kernels_on_plane_loop_rows(precision, n):
+    """Kernel widths above 8 (epsilon 1e-10: 16 taps per axis stored) on the
+    plane-loop row kernels, whole rows of 4096 (2048^2) and row parts of 4096
+    (8192^2), both directions, equal the rocFFT path."""
+    lib = _lib()
+    say(f"[wide kernels, plane-loop rows] {precision} {n}x{n}")
+    rng = np.random.default_rng(31)
+    f0, nrow, eps = 1e9, 3000, 1e-10
+    dx = 1.0 * np.pi / 180 / n
+    uvw = np.empty((nrow, 3))
+    uvw[:, :2] = (rng.random((nrow, 2)) - 0.5) * 0.98 / dx * C / f0
+    uvw[:, 2] = rng.uniform(-1, 1, nrow) * 10.0
+    wgt = rng.uniform(0.5, 1.0, nrow)
+    vis = rng.standard_normal(nrow) + 1j * rng.standard_normal(nrow)
+    img = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    info = _lib_plan(lib, nrow, n, n, 1.0, f0, eps, uvw, precision)
+    assert info["fft_lds"] == 1 and info["kw"] > 8
+    assert info["rsplit"] == (4 if n == 8192 else 0)
+    if precision == "fp64":
+        fd = lambda: gpu_ms2dirty(uvw, vis, wgt, n, n, 1.0, f0, eps)
+        fv = lambda: gpu_dirty2ms(uvw, img, wgt, n, n, 1.0, f0, eps)
+    else:
+        fd = lambda: gpu_f32(True, uvw, vis, wgt, n, n, 1.0, f0, eps)
+        fv = lambda: gpu_f32(False, uvw, img, wgt, n, n, 1.0, f0, eps)
+    d_lds, v_lds = fd(), fv()
+    assert lib.curig_set_fft_path(1) == 0
+    try:
+        d_roc, v_roc = fd(), fv()
+    finally:
+        lib.curig_set_fft_path(0)
+    e1, e2 = O.l2error(d_roc, d_lds), O.l2error(v_roc, v_lds)
+    say(f"  kw {info['kw']}: ms2dirty {e1:.2e}, dirty2ms {e2:.2e}")
+    tol = 1e-11 if precision == "fp64" else 2e-6
+    assert e1 < tol and e2 < tol
+
+
 def _lib_plan(lib, nrow, nx, ny, fov, f0, eps, uvw, precision):
     from curig_amd import _lib as L
 

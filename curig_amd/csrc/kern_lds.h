@@ -1146,16 +1146,20 @@ __global__ void __launch_bounds__(256) row_ifft_interp_part_planes_kernel(
     __syncthreads();  // the previous plane's interpolation is done with the LDS; this plane's walk table is complete
     const int total = wt[cur].total;
     if (total > 0) {
+      // the first record and (up to 8 taps per axis, where the registers
+      // allow) its taps are in flight during the transform
+      constexpr bool early = KWM <= 8;
       int idx0 = 0;
       Rec<T> r0;
+      Taps<T, KWM> q0;
       if (j < total) {
         idx0 = walk_record(wt[cur], j);
         r0 = recs[idx0];
+        if (early) load_taps<T, KWM>(taps, idx0, q0);
       }
       fft4096_part_regs_to_lds(lds, v, sign, pt);
       if (j < total) {
-        Taps<T, KWM> q0;
-        load_taps<T, KWM>(taps, idx0, q0);
+        if (!early) load_taps<T, KWM>(taps, idx0, q0);
         interp_row_part<T, KWM>(r0, q0, idx0, lds, tv, whs, g, p, row0, h, m - 1, sign, vacc);
       }
       for (int t = j + nt; t < total; t += nt) {
